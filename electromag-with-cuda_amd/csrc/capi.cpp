@@ -1,0 +1,388 @@
+// capi.cpp -- the extern "C" boundary of libemtrace.so (declared in
+// include/emtrace.h): argument checks with the reference's status codes,
+// launch-shape choice, the host-array drop-in with timed transfers, and
+// batch plans (sorted work items + persistent grid + optional hipGraph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "em_internal.hpp"
+
+namespace em {
+
+// ---------------------------------------------------------------------------
+// error state
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = 0; }
+
+// ---------------------------------------------------------------------------
+// device info
+static std::mutex g_mu;
+static DeviceInfo g_dev[64];
+
+int current_device_info(DeviceInfo& out)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        EM_FAIL(EM_ERR_NO_DEVICE, "no HIP device visible");
+    int dev = 0;
+    EM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) EM_FAIL(EM_ERR_NO_DEVICE, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceInfo& d = g_dev[dev];
+    if (d.device != dev) {
+        hipDeviceProp_t prop;
+        EM_HIP(hipGetDeviceProperties(&prop, dev));
+        d.cus = prop.multiProcessorCount;
+        d.lds_per_block = prop.sharedMemPerBlock;
+        std::snprintf(d.arch, sizeof(d.arch), "%s", prop.gcnArchName);
+        if (std::strncmp(d.arch, "gfx950", 6) != 0)
+            EM_FAIL(EM_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", dev,
+                    d.arch);
+        d.device = dev;
+    }
+    out = d;
+    return EM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch shape
+static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
+int validate_shape(const DeviceInfo& dev, const Shape& s)
+{
+    if (!pow2_in(s.S, 1, 32)) EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1..32}", s.S);
+    if (!pow2_in(s.block, 64, 512)) EM_FAIL(EM_ERR_INVALID, "block %d not in {64..512}", s.block);
+    if (s.tile == 0 || s.tile % (uint32_t)(s.S * kGroup) || s.tile % 256)
+        EM_FAIL(EM_ERR_INVALID, "tile %u not a positive multiple of 256 and of %d", s.tile, s.S * kGroup);
+    if ((size_t)s.tile * sizeof(float4) > dev.lds_per_block)
+        EM_FAIL(EM_ERR_INVALID, "tile %u x 16 B exceeds the %zu B of LDS per workgroup", s.tile,
+                dev.lds_per_block);
+    return EM_OK;
+}
+
+static uint32_t tile_for(const DeviceInfo& dev, uint64_t p, int S, int resident)
+{
+    const uint64_t G = (uint64_t)S * kGroup;
+    const uint64_t budget = dev.lds_per_block / (uint64_t)std::max(1, resident) / sizeof(float4);
+    const uint64_t Q = std::max<uint64_t>(256, G);  // tile granule
+    uint64_t tmax = std::max<uint64_t>(Q, budget / Q * Q);
+    const uint64_t need = (std::max<uint64_t>(p, 1) + Q - 1) / Q * Q;
+    if (need <= tmax) return (uint32_t)need;
+    const uint64_t tiles = (p + tmax - 1) / tmax;
+    const uint64_t t = ((p + tiles - 1) / tiles + Q - 1) / Q * Q;
+    return (uint32_t)std::min(t, tmax);
+}
+
+int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out)
+{
+    Shape s;
+    s.block = (req && req->block) ? req->block : 256;
+    if (req && req->lanes_per_line) {
+        s.S = req->lanes_per_line;
+    } else {
+        const uint64_t target_waves = 2ull * 4ull * (uint64_t)dev.cus;  // two waves per SIMD
+        int S = 1;
+        while (S < 32 && (n * S + 63) / 64 < target_waves && p / (2ull * S) >= 64) S *= 2;
+        s.S = S;
+    }
+    if (req && req->tile) {
+        s.tile = (uint32_t)req->tile;
+    } else {
+        const uint64_t blocks = (n * (uint64_t)s.S + s.block - 1) / s.block;
+        const int resident = (int)std::min<uint64_t>(std::max<uint64_t>(1, (blocks + dev.cus - 1) / dev.cus),
+                                                      std::max(1, 1024 / s.block));
+        s.tile = tile_for(dev, p, s.S, resident);
+    }
+    EM_TRY(validate_shape(dev, s));
+    out = s;
+    return EM_OK;
+}
+
+int validate_step(const em_step* st)
+{
+    if (!st) EM_FAIL(EM_ERR_INVALID, "step is NULL");
+    if (st->rule == kRulePlain || st->rule == kRuleRef) {
+        if (st->resolution == 0.f) EM_FAIL(EM_ERR_RESOLUTION, "resolution == 0");
+        if (!std::isfinite(st->resolution)) EM_FAIL(EM_ERR_INVALID, "resolution not finite");
+    } else if (st->rule == kRuleAngle) {
+        if (!(st->h0 > 0.f && st->hmin > 0.f && st->hmax >= st->hmin && st->turn > 0.f) ||
+            !std::isfinite(st->hmax) || !std::isfinite(st->h0))
+            EM_FAIL(EM_ERR_INVALID, "angle rule needs h0 > 0, 0 < hmin <= hmax, turn > 0");
+    } else {
+        EM_FAIL(EM_ERR_INVALID, "unknown step rule %d", st->rule);
+    }
+    return EM_OK;
+}
+
+StepP to_stepp(const em_step& st)
+{
+    StepP sp;
+    sp.rule = st.rule;
+    sp.res = st.resolution;
+    sp.h0 = st.h0;
+    sp.hmin = st.hmin;
+    sp.hmax = st.hmax;
+    sp.turn = st.turn;
+    sp.scale = (st.rule == kRuleAngle) ? 1.0f : (float)8.987551787E9;  // Electrostatics.h:27
+    return sp;
+}
+
+// common size checks, reference order: n, resolution, len
+static int check_sizes(size_t p, size_t n, size_t len, const em_step* st)
+{
+    if (!n) EM_FAIL(EM_ERR_NO_LINES, "n == 0");
+    EM_TRY(validate_step(st));
+    if (len < 2) EM_FAIL(EM_ERR_LENGTH, "len %zu < 2", len);
+    if (p > 0xFFFFFFFFull || n > 0xFFFFFFFFull || len > 0xFFFFFFFFull)
+        EM_FAIL(EM_ERR_INVALID, "p, n and len must each fit 32 bits");
+    return EM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device buffer cache for the host-array forms (one set per device)
+struct HostPathBuffers {
+    void* charges = nullptr;
+    size_t charges_bytes = 0;
+    float* lines = nullptr;  // 3 * len * n floats
+    size_t lines_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+static HostPathBuffers g_hp[64];
+
+static int ensure(void** ptr, size_t* have, size_t need)
+{
+    if (*have >= need && *ptr) return EM_OK;
+    if (*ptr) EM_HIP(hipFree(*ptr));
+    *ptr = nullptr;
+    *have = 0;
+    EM_HIP(hipMalloc(ptr, need));
+    *have = need;
+    return EM_OK;
+}
+
+static double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static void fill_perf(em_perf* perf, double p, double n, double len, double kernel_s)
+{
+    if (!perf) return;
+    perf->kernel_s = kernel_s;
+    perf->interactions = p * n * (len - 1.0);
+    perf->interactions_per_s = kernel_s > 0 ? perf->interactions / kernel_s : 0.0;
+    perf->line_steps_per_s = kernel_s > 0 ? n * (len - 1.0) / kernel_s : 0.0;
+}
+
+static int host_trace(const float* charges, size_t p, float* x, float* y, float* z, size_t n,
+                      size_t len, const em_step* st, const em_launch* launch, em_perf* perf)
+{
+    const double t0 = now_s();
+    if (perf) {
+        perf->progress = 0.0;
+        perf->time_s = perf->kernel_s = perf->h2d_s = perf->d2h_s = 0.0;
+        perf->h2d_bytes = perf->d2h_bytes = 0.0;
+        perf->gflops = 0.0;
+    }
+    EM_TRY(check_sizes(p, n, len, st));
+    if ((p && !charges) || !x || !y || !z) EM_FAIL(EM_ERR_INVALID, "NULL array");
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+    Shape sh;
+    EM_TRY(choose_shape(dev, p, n, launch, sh));
+
+    std::lock_guard<std::mutex> lk(g_mu);
+    HostPathBuffers& b = g_hp[dev.device];
+    if (!b.stream) {
+        EM_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        for (auto& e : b.ev) EM_HIP(hipEventCreate(&e));
+    }
+    const size_t cb = std::max<size_t>(p, 1) * sizeof(float4);
+    const size_t row = n * sizeof(float);
+    EM_TRY(ensure(&b.charges, &b.charges_bytes, cb));
+    EM_TRY(ensure(reinterpret_cast<void**>(&b.lines), &b.lines_bytes, 3 * len * row));
+    float* dx = b.lines;
+    float* dy = dx + len * n;
+    float* dz = dy + len * n;
+
+    EM_HIP(hipEventRecord(b.ev[0], b.stream));
+    if (p) EM_HIP(hipMemcpyAsync(b.charges, charges, p * sizeof(float4), hipMemcpyHostToDevice, b.stream));
+    EM_HIP(hipMemcpyAsync(dx, x, row, hipMemcpyHostToDevice, b.stream));
+    EM_HIP(hipMemcpyAsync(dy, y, row, hipMemcpyHostToDevice, b.stream));
+    EM_HIP(hipMemcpyAsync(dz, z, row, hipMemcpyHostToDevice, b.stream));
+    EM_HIP(hipEventRecord(b.ev[1], b.stream));
+
+    Cfg cfg{};
+    cfg.charge_off = 0;
+    cfg.line_off = 0;
+    cfg.pitch = n;
+    cfg.p = (uint32_t)p;
+    cfg.n = (uint32_t)n;
+    cfg.len = (uint32_t)len;
+    const StepP sp = to_stepp(*st);
+    EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
+                               nullptr, 1, b.stream));
+    EM_HIP(hipEventRecord(b.ev[2], b.stream));
+    const size_t out_bytes = (len - 1) * row;
+    EM_HIP(hipMemcpyAsync(x + n, dx + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
+    EM_HIP(hipMemcpyAsync(y + n, dy + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
+    EM_HIP(hipMemcpyAsync(z + n, dz + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
+    EM_HIP(hipEventRecord(b.ev[3], b.stream));
+    EM_HIP(hipEventSynchronize(b.ev[3]));
+
+    float ms[3] = {0, 0, 0};
+    EM_HIP(hipEventElapsedTime(&ms[0], b.ev[0], b.ev[1]));
+    EM_HIP(hipEventElapsedTime(&ms[1], b.ev[1], b.ev[2]));
+    EM_HIP(hipEventElapsedTime(&ms[2], b.ev[2], b.ev[3]));
+    if (perf) {
+        perf->time_s = now_s() - t0;
+        perf->h2d_s = ms[0] * 1e-3;
+        perf->d2h_s = ms[2] * 1e-3;
+        perf->h2d_bytes = (double)(p * sizeof(float4) + 3 * row);
+        perf->d2h_bytes = (double)(3 * out_bytes);
+        fill_perf(perf, (double)p, (double)n, (double)len, ms[1] * 1e-3);
+        // CPU_Implement.cpp:347-349
+        perf->gflops = ((double)n * ((double)len - 1) * ((double)p * 18.0 + 13.0)) / perf->time_s / 1e9;
+        perf->lanes_per_line = sh.S;
+        perf->tile = (int32_t)sh.tile;
+        perf->block = sh.block;
+        perf->progress = 1.0;
+    }
+    return EM_OK;
+}
+
+}  // namespace em
+
+using namespace em;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* em_last_error(void) { return g_err; }
+
+const char* em_version(void) { return "emtrace 0.1 (gfx950)"; }
+
+int em_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int em_init(int device)
+{
+    clear_error();
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        EM_FAIL(EM_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) EM_FAIL(EM_ERR_INVALID, "device %d of %d", device, count);
+    EM_HIP(hipSetDevice(device));
+    DeviceInfo d;
+    return current_device_info(d);
+}
+
+int em_shutdown(void)
+{
+    clear_error();
+    std::lock_guard<std::mutex> lk(g_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return EM_OK;
+    for (int i = 0; i < 64; ++i) {
+        HostPathBuffers& b = g_hp[i];
+        if (!b.stream && !b.charges && !b.lines) continue;
+        EM_HIP(hipSetDevice(i));
+        if (b.charges) EM_HIP(hipFree(b.charges));
+        if (b.lines) EM_HIP(hipFree(b.lines));
+        for (auto& e : b.ev)
+            if (e) EM_HIP(hipEventDestroy(e));
+        if (b.stream) EM_HIP(hipStreamDestroy(b.stream));
+        b = HostPathBuffers();
+    }
+    EM_HIP(hipSetDevice(cur));
+    return EM_OK;
+}
+
+int em_calc_lines_f32(const float* charges, size_t p, float* x, float* y, float* z, size_t n,
+                      size_t len, float resolution, int use_curvature, em_perf* perf)
+{
+    clear_error();
+    em_step st{};
+    st.rule = use_curvature ? EM_RULE_REF : EM_RULE_PLAIN;
+    st.resolution = resolution;
+    return host_trace(charges, p, x, y, z, n, len, &st, nullptr, perf);
+}
+
+int em_trace_host_f32(const float* charges, size_t p, float* x, float* y, float* z, size_t n,
+                      size_t len, const em_step* step, const em_launch* launch, em_perf* perf)
+{
+    clear_error();
+    return host_trace(charges, p, x, y, z, n, len, step, launch, perf);
+}
+
+size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every)
+{
+    if (!ckpt_every || len < 1) return 0;
+    return 8ull * n * ((len - 1) / ckpt_every + 1);
+}
+
+int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, float* d_z,
+                     size_t n, size_t len, size_t pitch, const em_step* step,
+                     const em_launch* launch, float* d_ckpt, uint32_t ckpt_every, void* stream)
+{
+    clear_error();
+    EM_TRY(check_sizes(p, n, len, step));
+    if ((p && !d_charges) || !d_x || !d_y || !d_z) EM_FAIL(EM_ERR_INVALID, "NULL device pointer");
+    if (pitch < n) EM_FAIL(EM_ERR_INVALID, "pitch %zu < n %zu", pitch, n);
+    if (d_ckpt && !ckpt_every) EM_FAIL(EM_ERR_INVALID, "checkpoint buffer given with ckpt_every = 0");
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+    Shape sh;
+    EM_TRY(choose_shape(dev, p, n, launch, sh));
+    Cfg cfg{};
+    cfg.pitch = pitch;
+    cfg.p = (uint32_t)p;
+    cfg.n = (uint32_t)n;
+    cfg.len = (uint32_t)len;
+    return launch_trace_direct(sh, cfg, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
+                               to_stepp(*step), d_ckpt, ckpt_every ? ckpt_every : 1,
+                               static_cast<hipStream_t>(stream));
+}
+
+int em_field_dev_f32(const float* d_charges, size_t p, const float* d_px, const float* d_py,
+                     const float* d_pz, size_t m, float scale, float* d_ex, float* d_ey,
+                     float* d_ez, const em_launch* launch, void* stream)
+{
+    clear_error();
+    if (!m) return EM_OK;
+    if ((p && !d_charges) || !d_px || !d_py || !d_pz || !d_ex || !d_ey || !d_ez)
+        EM_FAIL(EM_ERR_INVALID, "NULL device pointer");
+    if (p > 0xFFFFFFFFull) EM_FAIL(EM_ERR_INVALID, "p must fit 32 bits");
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+    Shape sh;
+    EM_TRY(choose_shape(dev, p, m, launch, sh));
+    return launch_field(sh, reinterpret_cast<const float4*>(d_charges), (uint32_t)p, d_px, d_py,
+                        d_pz, m, scale, d_ex, d_ey, d_ez, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

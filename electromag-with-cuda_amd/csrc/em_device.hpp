@@ -1,0 +1,259 @@
+// em_device.hpp -- device-side building blocks of the gfx950 field-line tracer.
+//
+// Layout and work split (CDNA4, wave64):
+//   * one line is owned by S adjacent lanes (S = 1..32); the S lanes split the
+//     charge sum (lane k takes charges k, k+S, k+2S, ... of every tile) and
+//     combine it with an xor butterfly, so every lane of the group ends the
+//     step holding the same bits;
+//   * charges are staged through LDS in tiles of float4 {x, y, z, q}; inside a
+//     tile all lanes of a wave read one of S consecutive records -- a
+//     broadcast from LDS with no bank conflict for S <= 8;
+//   * the Coulomb term is FP32 with v_rsq_f32 (12 VALU + 1 transcendental per
+//     charge-point interaction); accumulation is blocked: plain FP32 partial
+//     sums over <= kBlk charges, folded into a (sum, compensation) pair with
+//     Knuth's TwoSum;
+//   * the step rule and the coalesced row store run in the same kernel.
+//
+// This file is compiled with -ffp-contract=off: every fused multiply-add in
+// the hot loop is written out as __builtin_fmaf, and the step arithmetic
+// below is evaluated exactly in source order, which is the order of the
+// reference's vec3 helpers (include/Vector.h:158-197).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace em {
+
+constexpr int kUnroll = 8;    // charges per lane per unrolled group
+constexpr int kGroup = 2 * kUnroll;  // padding granule per lane (two pipelined groups)
+constexpr int kBlk = 128;     // charges per lane per compensated block (multiple of kGroup)
+constexpr float kFar = 1.0e15f;  // padding charge position (q = 0)
+
+enum : int { kRulePlain = 0, kRuleRef = 1, kRuleAngle = 2 };
+
+// Internal per-configuration descriptor (the public em_batch_cfg plus the
+// offsets of the adjoint's buffers).
+struct Cfg {
+    uint64_t charge_off;  // first charge record
+    uint64_t line_off;    // element offset of (row 0, line 0) in x/y/z
+    uint64_t pitch;       // row pitch (elements)
+    uint64_t ckpt_off;    // first float of this configuration's checkpoints
+    uint64_t src_off;     // first record of this configuration's adjoint sources
+    uint32_t p, n, len, grad_line_off;
+};
+
+struct Item {
+    uint32_t cfg;
+    uint32_t line0;
+};
+
+struct StepP {
+    int rule;
+    float res, h0, hmin, hmax, turn;
+    float scale;  // factor applied to the Coulomb sum (k_e for plain/ref)
+};
+
+__host__ __device__ constexpr uint32_t round_up(uint32_t a, uint32_t b) { return (a + b - 1) / b * b; }
+
+// One charge-point interaction: E += q (p - c) / |p - c|^3.
+__device__ __forceinline__ void coulomb(const float4 c, float px, float py, float pz,
+                                        float& ex, float& ey, float& ez)
+{
+    const float dx = px - c.x;
+    const float dy = py - c.y;
+    const float dz = pz - c.z;
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float s = (c.w * inv) * (inv * inv);
+    ex = __builtin_fmaf(s, dx, ex);
+    ey = __builtin_fmaf(s, dy, ey);
+    ez = __builtin_fmaf(s, dz, ez);
+}
+
+// Knuth TwoSum: (sum, comp) += part, error-free.
+__device__ __forceinline__ void fold(float& sum, float& comp, float part)
+{
+    const float t = sum + part;
+    const float bp = t - sum;
+    const float err = (sum - (t - bp)) + (part - bp);
+    sum = t;
+    comp += err;
+}
+
+struct Acc {
+    float sx, sy, sz, cx, cy, cz;
+    __device__ void zero() { sx = sy = sz = cx = cy = cz = 0.f; }
+};
+
+// Sum the charges of one LDS tile into acc.  `count` is a multiple of
+// S*kGroup (the tile is padded with q = 0 records far away).  Software-
+// pipelined: while one group of kUnroll charges is being summed, the LDS reads
+// of the next group are in flight (two register sets, no copies).
+template <int S>
+__device__ __forceinline__ void sum_tile(const float4* __restrict__ tile, uint32_t count, int k,
+                                         float px, float py, float pz, Acc& acc)
+{
+    const uint32_t ngroups = count / (S * kUnroll);  // even: count is a multiple of S*kGroup
+    const float4* t = tile + k;
+    constexpr uint32_t kStride = S * kUnroll;
+    constexpr uint32_t kFoldEvery = kBlk / kUnroll;  // groups per compensated block (even)
+    float4 a[kUnroll], b[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) a[u] = t[u * S];
+    float ex = 0.f, ey = 0.f, ez = 0.f;
+    for (uint32_t g = 0; g < ngroups; g += 2) {
+        // sched_barrier(0) keeps each group's LDS reads issued ahead of the
+        // other group's arithmetic (the scheduler would otherwise sink them
+        // next to their uses and expose the LDS latency every iteration).
+        const float4* tb = t + (g + 1) * kStride;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) b[u] = tb[u * S];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) coulomb(a[u], px, py, pz, ex, ey, ez);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t gn = min(g + 2, ngroups - 2);  // last pair re-reads itself
+        const float4* ta = t + gn * kStride;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) a[u] = ta[u * S];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) coulomb(b[u], px, py, pz, ex, ey, ez);
+        __builtin_amdgcn_sched_barrier(0);
+        if ((g + 2) % kFoldEvery == 0 || g + 2 >= ngroups) {
+            fold(acc.sx, acc.cx, ex);
+            fold(acc.sy, acc.cy, ey);
+            fold(acc.sz, acc.cz, ez);
+            ex = ey = ez = 0.f;
+        }
+    }
+}
+
+// Cooperative global -> LDS copy of `cnt` charge records, padded to `cntp`.
+__device__ __forceinline__ void stage(float4* __restrict__ tile, const float4* __restrict__ src,
+                                      uint32_t cnt, uint32_t cntp)
+{
+    for (uint32_t i = threadIdx.x; i < cntp; i += blockDim.x)
+        tile[i] = i < cnt ? src[i] : make_float4(kFar, kFar, kFar, 0.f);
+}
+
+// Butterfly sum over the S lanes of a line group; identical bits in all S lanes.
+template <int S>
+__device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+    for (int m = S / 2; m >= 1; m /= 2) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Full Coulomb sum at (px, py, pz) over p charges, using the LDS tile.
+// If p fits the tile (pp <= tile_cap) the tile is assumed already staged.
+// Every thread of the block must call this (it contains barriers when the
+// charges do not fit).
+template <int S>
+__device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t tile_cap,
+                                         const float4* __restrict__ charges, uint32_t p,
+                                         int k, float px, float py, float pz,
+                                         float& ex, float& ey, float& ez)
+{
+    Acc acc;
+    acc.zero();
+    constexpr uint32_t G = S * kGroup;
+    const uint32_t pp = round_up(p, G);
+    if (pp <= tile_cap) {
+        sum_tile<S>(tile, pp, k, px, py, pz, acc);
+    } else {
+        for (uint32_t t0 = 0; t0 < p; t0 += tile_cap) {
+            const uint32_t cnt = min(tile_cap, p - t0);
+            const uint32_t cntp = round_up(cnt, G);
+            __syncthreads();
+            stage(tile, charges + t0, cnt, cntp);
+            __syncthreads();
+            sum_tile<S>(tile, cntp, k, px, py, pz, acc);
+        }
+    }
+    ex = group_sum<S>(acc.sx + acc.cx);
+    ey = group_sum<S>(acc.sy + acc.cy);
+    ez = group_sum<S>(acc.sz + acc.cz);
+}
+
+// Per-line state carried across steps.
+struct LineState {
+    float px, py, pz;  // current point x_{s-1}
+    float ux, uy, uz;  // previous unit direction (angle rule)
+    float h;           // previous step length (angle rule)
+    float Px, Py, Pz;  // seed point (ref rule: the SSE path's prevAccum)
+};
+
+// Record of one step, kept by the adjoint's recompute.
+struct StepRec {
+    float L;       // |E|
+    float c;       // chord |u - u_prev|
+    float h;       // step taken
+    int branch;    // 0: turn ratio, 1: clamp hmax, 2: clamp hmin, 3: constant
+    float ux, uy, uz;
+};
+
+// Advance one step.  E must already carry sp.scale.  Operation order is that
+// of oracle/restate.c (and, for plain/ref, of the reference's vec3 helpers).
+__device__ __forceinline__ void step_update(const StepP& sp, uint32_t s, float ex, float ey,
+                                            float ez, LineState& st, StepRec* rec = nullptr)
+{
+    if (sp.rule == kRulePlain) {
+        // CPU_Implement.cpp:89-90 with Vector.h:158-164
+        const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+        const float sc = sp.res * len;
+        st.px = st.px + ex / sc;
+        st.py = st.py + ey / sc;
+        st.pz = st.pz + ez / sc;
+        if (rec) {
+            rec->L = len; rec->c = 0.f; rec->h = 1.f / sp.res; rec->branch = 3;
+            rec->ux = ex / len; rec->uy = ey / len; rec->uz = ez / len;
+        }
+    } else if (sp.rule == kRuleRef) {
+        // CPU_Implement.cpp:327-331 (P = seed point, never updated)
+        float k = ex * ex + ey * ey + ez * ez;
+        const float ix = ex - st.Px, iy = ey - st.Py, iz = ez - st.Pz;
+        const float cx = iy * st.Pz - iz * st.Py;
+        const float cy = iz * st.Px - ix * st.Pz;
+        const float cz = ix * st.Py - iy * st.Px;
+        k = sqrtf(cx * cx + cy * cy + cz * cz) / (k * sqrtf(k));
+        float sc = (k + 1.f) * sp.res;
+        sc = sc * sqrtf(ex * ex + ey * ey + ez * ez);
+        st.px = st.px + ex / sc;
+        st.py = st.py + ey / sc;
+        st.pz = st.pz + ez / sc;
+    } else {
+        const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+        const float ux = ex / len, uy = ey / len, uz = ez / len;
+        float h = sp.h0, c = 0.f;
+        int branch = 3;
+        if (s > 1) {
+            const float dx = ux - st.ux, dy = uy - st.uy, dz = uz - st.uz;
+            c = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float th = sp.turn * st.h;
+            if (c * sp.hmax > th) {
+                h = th / c;
+                branch = 0;
+            } else {
+                h = sp.hmax;
+                branch = 1;
+            }
+            if (!(h > sp.hmin)) {
+                h = sp.hmin;
+                branch = 2;
+            }
+        }
+        st.px = st.px + h * ux;
+        st.py = st.py + h * uy;
+        st.pz = st.pz + h * uz;
+        st.ux = ux; st.uy = uy; st.uz = uz; st.h = h;
+        if (rec) {
+            rec->L = len; rec->c = c; rec->h = h; rec->branch = branch;
+            rec->ux = ux; rec->uy = uy; rec->uz = uz;
+        }
+    }
+}
+
+}  // namespace em

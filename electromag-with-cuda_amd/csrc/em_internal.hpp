@@ -1,0 +1,79 @@
+// em_internal.hpp -- host-side internals shared by the translation units of
+// libemtrace.so (error state, HIP checks, device properties, launch choice).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/emtrace.h"
+#include "em_device.hpp"
+
+namespace em {
+
+// ---- error state (thread-local message, C-ABI status codes) ----------------
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define EM_FAIL(code, ...)            \
+    do {                              \
+        ::em::set_error(__VA_ARGS__); \
+        return (code);                \
+    } while (0)
+
+#define EM_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            EM_FAIL(e_ == hipErrorOutOfMemory ? EM_ERR_NOMEM : EM_ERR_HIP, "%s: %s (%s:%d)", \
+                    #call, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+#define EM_TRY(call)            \
+    do {                        \
+        int rc_ = (call);       \
+        if (rc_) return rc_;    \
+    } while (0)
+
+// ---- device ----------------------------------------------------------------
+struct DeviceInfo {
+    int device = -1;
+    int cus = 0;
+    size_t lds_per_block = 0;
+    char arch[64] = {0};
+};
+int current_device_info(DeviceInfo& out);  // inits lazily; EM_ERR_NO_DEVICE if none
+
+// ---- launch shape ----------------------------------------------------------
+struct Shape {
+    int S = 1;         // lanes per line
+    int block = 256;   // threads per workgroup
+    uint32_t tile = 0; // charges per LDS tile
+};
+
+// Heuristic launch shape for one configuration (p charges, n lines); fields
+// set in `req` (non-zero) override it.  Fails with EM_ERR_INVALID on an
+// unsupported request.
+int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out);
+int validate_shape(const DeviceInfo& dev, const Shape& s);
+
+StepP to_stepp(const em_step& st);
+int validate_step(const em_step* st);
+
+// ---- kernels (defined in trace.hip / adjoint.hip) --------------------------
+int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
+                        float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
+                        hipStream_t stream);
+int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const Item* items,
+                            uint32_t n_items, uint32_t* counter, const float4* charges, float* x,
+                            float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
+                            hipStream_t stream);
+int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid);
+int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,
+                 const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,
+                 float* ez, hipStream_t stream);
+
+}  // namespace em

@@ -1,0 +1,265 @@
+"""Python-level API over the C ABI (numpy for host arrays, torch for device memory).
+
+Host-array calls mirror the reference's calc-lines interface
+(``CalcField_CPU<float>``, ElectroMag/src/CPU Implement.h:25-30): charges are
+``(p, 4)`` float32 records, lines are three step-major ``len*n`` arrays with the
+seeds in row 0.  Here they are carried as one ``(3, len, n)`` array.
+
+Device calls take torch tensors on the current HIP device and run on torch's
+current stream; torch is only the allocator and stream provider.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, check, em_batch_cfg, em_launch, em_perf, em_step, lib
+from .rules import ANGLE, PLAIN, REF, StepRule
+
+
+def make_step(rule: StepRule) -> em_step:
+    return em_step(int(rule.rule), float(rule.resolution), float(rule.h0), float(rule.hmin),
+                   float(rule.hmax), float(rule.turn))
+
+
+def make_launch(lanes_per_line: int = 0, tile: int = 0, block: int = 0) -> em_launch:
+    return em_launch(int(lanes_per_line), int(tile), int(block))
+
+
+def _launch_arg(launch):
+    if launch is None:
+        return None
+    if isinstance(launch, em_launch):
+        return ctypes.byref(launch)
+    if isinstance(launch, dict):
+        return ctypes.byref(make_launch(**launch))
+    return ctypes.byref(make_launch(*launch))
+
+
+def _np_ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# host arrays
+def calc_lines(charges: np.ndarray, x: np.ndarray, y: np.ndarray, z: np.ndarray, n: int,
+               resolution: float, use_curvature: bool = False):
+    """Drop-in for ``CalcField_CPU<float>``: fills rows 1.. of x/y/z in place.
+
+    Returns ``(status, perf)`` -- the status is the reference's int code
+    (0 ok, 1 n == 0, 2 resolution == 0, 3 fewer than two rows)."""
+    charges = np.ascontiguousarray(charges, dtype=np.float32)
+    for a in (x, y, z):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise TypeError("x, y, z must be contiguous float32 arrays (written in place)")
+    length = x.size // n if n else 0
+    perf = em_perf()
+    rc = lib().em_calc_lines_f32(_np_ptr(charges), charges.shape[0], _np_ptr(x), _np_ptr(y),
+                                 _np_ptr(z), n, length, float(resolution), int(bool(use_curvature)),
+                                 ctypes.byref(perf))
+    return rc, perf.as_dict()
+
+
+def trace_host(charges: np.ndarray, seeds: np.ndarray, length: int, rule: StepRule, launch=None):
+    """Trace on the GPU from host arrays.  Returns (rows[3, len, n], perf dict)."""
+    charges = np.ascontiguousarray(charges, dtype=np.float32)
+    seeds = np.asarray(seeds, dtype=np.float32)
+    n = seeds.shape[0]
+    rows = np.zeros((3, length, n), dtype=np.float32)
+    rows[:, 0, :] = seeds.T
+    perf = em_perf()
+    st = make_step(rule)
+    call("em_trace_host_f32", _np_ptr(charges), charges.shape[0], _np_ptr(rows[0]),
+         _np_ptr(rows[1]), _np_ptr(rows[2]), n, length, ctypes.byref(st), _launch_arg(launch),
+         ctypes.byref(perf))
+    return rows, perf.as_dict()
+
+
+# ---------------------------------------------------------------------------
+# device tensors
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev_check(t, name, dtype=None):
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError(f"{name} must be a torch tensor on the GPU")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}")
+
+
+def seed_rows(seeds, length: int, pitch: Optional[int] = None):
+    """Allocate a (3, len, pitch) float32 device tensor with the seeds in row 0."""
+    torch = _torch()
+    n = seeds.shape[0]
+    pitch = n if pitch is None else pitch
+    rows = torch.empty((3, length, pitch), dtype=torch.float32, device=seeds.device)
+    rows[:, 0, :n] = seeds.t()
+    return rows
+
+
+def ckpt_size(n: int, length: int, every: int) -> int:
+    return int(lib().em_ckpt_floats(n, length, every))
+
+
+def trace_into(charges, rows, n: int, rule: StepRule, launch=None, ckpt=None, ckpt_every: int = 0):
+    """Trace in place on device rows (3, len, pitch); row 0 must hold the seeds."""
+    torch = _torch()
+    _dev_check(charges, "charges", torch.float32)
+    _dev_check(rows, "rows", torch.float32)
+    if charges.dim() != 2 or charges.shape[1] != 4:
+        raise ValueError("charges must be (p, 4)")
+    if rows.dim() != 3 or rows.shape[0] != 3:
+        raise ValueError("rows must be (3, len, pitch)")
+    _, length, pitch = rows.shape
+    st = make_step(rule)
+    cp = None
+    if ckpt is not None:
+        _dev_check(ckpt, "ckpt", torch.float32)
+        if ckpt.numel() < ckpt_size(n, length, ckpt_every):
+            raise ValueError("checkpoint buffer too small")
+        cp = ckpt.data_ptr()
+    call("em_trace_dev_f32", charges.data_ptr(), charges.shape[0], rows[0].data_ptr(),
+         rows[1].data_ptr(), rows[2].data_ptr(), n, length, pitch, ctypes.byref(st),
+         _launch_arg(launch), cp, ckpt_every, _stream_handle())
+    return rows
+
+
+def trace(charges, seeds, length: int, rule: StepRule, launch=None):
+    """Trace from device seeds (n, 3).  Returns rows (3, len, n) on the device."""
+    rows = seed_rows(seeds, length)
+    return trace_into(charges, rows, seeds.shape[0], rule, launch)
+
+
+def field(charges, points, scale: float = 1.0, launch=None):
+    """E at device points (m, 3) -> (m, 3): scale * sum q (x - c)/|x - c|^3."""
+    torch = _torch()
+    _dev_check(charges, "charges", torch.float32)
+    pts = points.t().contiguous().float()
+    m = pts.shape[1]
+    E = torch.empty((3, m), dtype=torch.float32, device=points.device)
+    call("em_field_dev_f32", charges.data_ptr(), charges.shape[0], pts[0].data_ptr(),
+         pts[1].data_ptr(), pts[2].data_ptr(), m, float(scale), E[0].data_ptr(),
+         E[1].data_ptr(), E[2].data_ptr(), _launch_arg(launch), _stream_handle())
+    return E.t()
+
+
+# ---------------------------------------------------------------------------
+# batched form
+@dataclass
+class BatchLayout:
+    """Packing of many configurations: charges (sum p, 4); lines (3, total)."""
+    p: np.ndarray
+    n: np.ndarray
+    len: np.ndarray
+    charge_off: np.ndarray
+    line_off: np.ndarray
+    total_charges: int
+    total_line_elems: int
+
+    @staticmethod
+    def of(ps: Sequence[int], ns: Sequence[int], lens: Sequence[int]) -> "BatchLayout":
+        p = np.asarray(ps, dtype=np.int64)
+        n = np.asarray(ns, dtype=np.int64)
+        ln = np.asarray(lens, dtype=np.int64)
+        coff = np.concatenate([[0], np.cumsum(p)[:-1]])
+        sizes = n * ln
+        loff = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+        return BatchLayout(p, n, ln, coff, loff, int(p.sum()), int(sizes.sum()))
+
+    def cfg_array(self):
+        arr = (em_batch_cfg * len(self.p))()
+        for i in range(len(self.p)):
+            arr[i] = em_batch_cfg(int(self.charge_off[i]), int(self.line_off[i]), int(self.n[i]),
+                                  int(self.p[i]), int(self.n[i]), int(self.len[i]), 0)
+        return arr
+
+    def rows_view(self, lines, i: int):
+        """(3, len_i, n_i) view of configuration i inside the packed lines."""
+        o, n, ln = int(self.line_off[i]), int(self.n[i]), int(self.len[i])
+        return lines[:, o:o + n * ln].reshape(3, ln, n)
+
+    def interactions(self) -> int:
+        return int((self.p * self.n * (self.len - 1)).sum())
+
+
+class BatchPlan:
+    """Persistent batched launch over a BatchLayout (optionally hipGraph-replayed)."""
+
+    def __init__(self, layout: BatchLayout, rule: StepRule, launch=None, use_graph: bool = True):
+        self.layout = layout
+        self._cfgs = layout.cfg_array()
+        self._step = make_step(rule)
+        h = ctypes.c_void_p()
+        call("em_batch_create", self._cfgs, len(layout.p), ctypes.byref(self._step),
+             _launch_arg(launch), int(use_graph), ctypes.byref(h))
+        self._h = h
+
+    def info(self):
+        inter = ctypes.c_double()
+        items = ctypes.c_int32()
+        grid = ctypes.c_int32()
+        used = em_launch()
+        call("em_batch_info", self._h, ctypes.byref(inter), ctypes.byref(items), ctypes.byref(grid),
+             ctypes.byref(used))
+        return {"interactions": inter.value, "items": items.value, "grid": grid.value,
+                "lanes_per_line": used.lanes_per_line, "tile": used.tile, "block": used.block}
+
+    def run(self, charges, lines):
+        torch = _torch()
+        _dev_check(charges, "charges", torch.float32)
+        _dev_check(lines, "lines", torch.float32)
+        if charges.shape[0] < self.layout.total_charges or lines.shape[1] < self.layout.total_line_elems:
+            raise ValueError("packed buffers smaller than the layout")
+        call("em_batch_run", self._h, charges.data_ptr(), lines[0].data_ptr(), lines[1].data_ptr(),
+             lines[2].data_ptr(), _stream_handle())
+
+    def ckpt_size(self, every: int) -> int:
+        return int(lib().em_batch_ckpt_floats(self._h, every))
+
+    def run_ckpt(self, charges, lines, ckpt, every: int):
+        call("em_batch_run_ckpt", self._h, charges.data_ptr(), lines[0].data_ptr(),
+             lines[1].data_ptr(), lines[2].data_ptr(), ckpt.data_ptr(), every, _stream_handle())
+
+    def adjoint(self, charges, lines, ckpt, every: int, grad_lines, grad_charges):
+        call("em_batch_adjoint", self._h, charges.data_ptr(), lines[0].data_ptr(),
+             lines[1].data_ptr(), lines[2].data_ptr(), ckpt.data_ptr(), every,
+             grad_lines[0].data_ptr(), grad_lines[1].data_ptr(), grad_lines[2].data_ptr(),
+             grad_charges.data_ptr(), _stream_handle())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(lib().em_batch_destroy(self._h), "em_batch_destroy")
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_batch(workloads, device=None):
+    """Pack a list of gen.Workload into (layout, charges, lines) device tensors."""
+    torch = _torch()
+    layout = BatchLayout.of([w.p for w in workloads], [w.n for w in workloads],
+                            [w.len for w in workloads])
+    charges = torch.from_numpy(np.concatenate([w.charges for w in workloads])).to(device or "cuda")
+    lines = torch.empty((3, layout.total_line_elems), dtype=torch.float32, device=charges.device)
+    for i, w in enumerate(workloads):
+        v = layout.rows_view(lines, i)
+        v[:, 0, :] = torch.from_numpy(w.seeds.T.copy()).to(charges.device)
+    return layout, charges, lines

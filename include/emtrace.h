@@ -1,0 +1,180 @@
+/*
+ * emtrace.h -- C ABI of the MI355X (gfx950) field-line tracer.
+ *
+ * Plain C: plain pointers and sizes, no C++ or torch types, no exceptions
+ * across the boundary.  Every entry point returns an int status (0 = success,
+ * codes below) and leaves a thread-local message for em_last_error().
+ *
+ * Data model (same as the reference's calc-lines interface):
+ *   charges : p records {x, y, z, q} of float32, 16 bytes each -- the layout
+ *             of electro::pointCharge<float> (include/Electrostatics.h:30-35).
+ *   lines   : three float32 arrays x, y, z, step-major: element (line, step)
+ *             at [step * pitch + line]; row 0 holds the seeds (input), rows
+ *             1..len-1 are written (ElectroMag/src/CPU_Implement.cpp:80,92,
+ *             334-337; include/SOA_utils.hpp:31-112).  pitch == n in the
+ *             reference; the device forms accept pitch >= n.
+ */
+#ifndef EMTRACE_H
+#define EMTRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* 1..3 keep the reference's meaning (CPU_Implement.cpp:43-52, 213-230). */
+#define EM_OK 0
+#define EM_ERR_NO_LINES 1      /* n == 0                                     */
+#define EM_ERR_RESOLUTION 2    /* resolution == 0 (plain / ref rules)        */
+#define EM_ERR_LENGTH 3        /* fewer than 2 rows per line                 */
+/* 16 and up are new: the reference reports device failures only as text.   */
+#define EM_ERR_INVALID 16      /* null pointer, bad size or parameter        */
+#define EM_ERR_HIP 17          /* HIP runtime / launch failure               */
+#define EM_ERR_NOMEM 18        /* device allocation failed                   */
+#define EM_ERR_NO_DEVICE 19    /* no usable gfx950 device                    */
+#define EM_ERR_UNSUPPORTED 20  /* combination not implemented (see message)  */
+
+/* ---- step rules -------------------------------------------------------- */
+#define EM_RULE_PLAIN 0 /* x += E/(|E| res)               CPU_Implement.cpp:89-90   */
+#define EM_RULE_REF 1   /* x += E/(|E| (k+1) res), k = |(E-P)xP|/|E|^3, P = seed,
+                           E with k_e                      CPU_Implement.cpp:322-331 */
+#define EM_RULE_ANGLE 2 /* h = clamp(turn*h_prev/|u - u_prev|, hmin, hmax), h_1 = h0
+                           (project-defined; no reference counterpart)              */
+
+typedef struct em_step {
+    int32_t rule;
+    float resolution; /* plain / ref: step length 1/resolution          */
+    float h0;         /* angle: first step                              */
+    float hmin, hmax; /* angle: clamp                                   */
+    float turn;       /* angle: target chord between successive directions */
+} em_step;
+
+/* Launch shape.  Any field left 0 is chosen by the library (autotuner cache,
+ * else a heuristic from the problem size and the device's CU count). */
+typedef struct em_launch {
+    int32_t lanes_per_line; /* 1, 2, 4, 8, 16 or 32 lanes split one line's charge sum */
+    int32_t tile;           /* charges per LDS tile (multiple of 64)                  */
+    int32_t block;          /* threads per workgroup: 64, 128, 256 or 512             */
+} em_launch;
+
+/* Timing/performance record -- the fields of the reference's perfPacket
+ * (include/Data Structures.h:221-236: performance [GFLOP/s], time [s],
+ * progress) plus the per-phase times of CL_Electrostatics.cpp:472-511. */
+typedef struct em_perf {
+    double time_s;             /* whole call, wall clock                       */
+    double kernel_s;           /* HIP-event time of the tracing kernel         */
+    double h2d_s, d2h_s;       /* host<->device copies (host-array forms)      */
+    double h2d_bytes, d2h_bytes;
+    double interactions;       /* p * n * (len-1) charge-point interactions    */
+    double interactions_per_s; /* interactions / kernel_s                      */
+    double line_steps_per_s;   /* n * (len-1) / kernel_s                       */
+    double gflops;             /* n*(len-1)*(p*18+13)/time_s/1e9, the reference
+                                  formula (CPU_Implement.cpp:347-349)          */
+    volatile double progress;  /* 0 at entry, 1.0 when the rows are on the host */
+    int32_t lanes_per_line, tile, block; /* launch shape actually used     */
+} em_perf;
+
+/* ---- library lifetime ----------------------------------------------------*/
+int em_init(int device);           /* optional; entry points init lazily  */
+int em_shutdown(void);             /* frees cached device buffers          */
+const char* em_last_error(void);   /* thread-local, never NULL             */
+int em_device_count(void);         /* HIP devices visible (0 without GPU)  */
+const char* em_version(void);
+
+/* ---- drop-in for CalcField_CPU<float> ------------------------------------
+ * Replaces: template<> int CalcField_CPU<float>(Vector3<Array<float>>&,
+ *   Array<pointCharge<float>>&, size_t n, float resolution, perfPacket&,
+ *   bool useCurvature)   (ElectroMag/src/CPU Implement.h:25-30,
+ *   CPU_Implement.cpp:501-513).  Host arrays, synchronous; x/y/z hold len*n
+ * floats with row 0 = seeds.  use_curvature selects EM_RULE_REF, else
+ * EM_RULE_PLAIN.  Unlike the SSE path it does not require n % 64 == 0. */
+int em_calc_lines_f32(const float* charges, size_t p, float* x, float* y,
+                      float* z, size_t n, size_t len, float resolution,
+                      int use_curvature, em_perf* perf);
+
+/* Host arrays, any step rule and launch shape (launch may be NULL). */
+int em_trace_host_f32(const float* charges, size_t p, float* x, float* y,
+                      float* z, size_t n, size_t len, const em_step* step,
+                      const em_launch* launch, em_perf* perf);
+
+/* ---- device-pointer forms (asynchronous on `stream`, a hipStream_t) -------
+ * Replaces the OpenCL functor's device path (GPGPU_Segment/src/
+ * CL_Electrostatics.cpp:396-513 MainFunctor; Electrostatics.cl.c:282-367).
+ * d_ckpt (may be NULL): when non-NULL, the forward also stores the state
+ * needed by em_adjoint_dev_f32 every ckpt_every steps; size it with
+ * em_ckpt_floats(). */
+int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y,
+                     float* d_z, size_t n, size_t len, size_t pitch,
+                     const em_step* step, const em_launch* launch,
+                     float* d_ckpt, uint32_t ckpt_every, void* stream);
+
+size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every);
+
+/* E(x) = scale * sum_i q_i (x - c_i)/|x - c_i|^3 at m points (SoA px/py/pz),
+ * written to ex/ey/ez.  The same device code as the tracer's charge loop
+ * (include/Electrostatics.h:38-60 summed over all charges). */
+int em_field_dev_f32(const float* d_charges, size_t p, const float* d_px,
+                     const float* d_py, const float* d_pz, size_t m,
+                     float scale, float* d_ex, float* d_ey, float* d_ez,
+                     const em_launch* launch, void* stream);
+
+/* ---- batched form: many independent configurations, one persistent launch
+ * No reference counterpart (the reference traces one configuration per call).
+ * A plan sorts (configuration, line-block) work items by cost and sizes a
+ * persistent grid; em_batch_run launches it (the first run of a captured plan
+ * records a hipGraph, later runs replay it). */
+typedef struct em_batch_cfg {
+    uint64_t charge_offset; /* first charge record of this configuration     */
+    uint64_t line_offset;   /* element offset of (row 0, line 0) in x/y/z    */
+    uint64_t pitch;         /* row pitch in elements (>= n)                  */
+    uint32_t p;             /* charges                                       */
+    uint32_t n;             /* lines                                         */
+    uint32_t len;           /* rows (>= 2)                                   */
+    uint32_t reserved;
+} em_batch_cfg;
+
+typedef struct em_batch em_batch; /* opaque */
+
+int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
+                    const em_launch* launch, int use_graph, em_batch** out);
+int em_batch_run(em_batch* plan, const float* d_charges, float* d_x, float* d_y,
+                 float* d_z, void* stream);
+int em_batch_info(const em_batch* plan, double* interactions, int32_t* items,
+                  int32_t* grid, em_launch* used);
+int em_batch_destroy(em_batch* plan);
+
+/* ---- adjoint --------------------------------------------------------------
+ * No reference counterpart.  Given the forward's rows and checkpoints
+ * (d_ckpt from em_trace_dev_f32 with the same ckpt_every) and the loss
+ * gradient G_s = dL/dx_s for every row (gx/gy/gz, same layout as x/y/z),
+ * recomputes each checkpoint segment, sweeps it backward, and writes
+ *   d_grad_charges : p x 4 floats, dL/d{x, y, z, q} of every charge,
+ *   d_grad_seeds   : 3 x n floats (SoA), dL/d(seed point)  (may be NULL).
+ * Rules: EM_RULE_PLAIN and EM_RULE_ANGLE (EM_RULE_REF -> EM_ERR_UNSUPPORTED). */
+int em_adjoint_dev_f32(const float* d_charges, size_t p, const float* d_x,
+                       const float* d_y, const float* d_z, size_t n, size_t len,
+                       size_t pitch, const em_step* step, const em_launch* launch,
+                       const float* d_ckpt, uint32_t ckpt_every,
+                       const float* d_gx, const float* d_gy, const float* d_gz,
+                       float* d_grad_charges, float* d_grad_seeds, void* stream);
+
+/* Batched adjoint over the configurations of a plan (charges, rows and
+ * gradients packed as for em_batch_run; checkpoints from
+ * em_batch_run_ckpt; grad arrays indexed like charges / lines). */
+size_t em_batch_ckpt_floats(const em_batch* plan, uint32_t ckpt_every);
+int em_batch_run_ckpt(em_batch* plan, const float* d_charges, float* d_x,
+                      float* d_y, float* d_z, float* d_ckpt,
+                      uint32_t ckpt_every, void* stream);
+int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x,
+                     const float* d_y, const float* d_z, const float* d_ckpt,
+                     uint32_t ckpt_every, const float* d_gx, const float* d_gy,
+                     const float* d_gz, float* d_grad_charges, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMTRACE_H */

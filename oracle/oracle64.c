@@ -1,0 +1,311 @@
+/*
+ * oracle/oracle64.c -- TEST INFRASTRUCTURE ONLY (the FP64 oracle).
+ *
+ * Double-precision statements of the physics the tracer computes, used by
+ * tests/ as the checker for the HIP kernels.  Nothing in the product library
+ * links, calls or loads this file.
+ *
+ *   o64_field      E(x) = s * sum_i q_i (x - c_i)/|x - c_i|^3, Neumaier-
+ *                  compensated double sum (the Coulomb sum of
+ *                  include/Electrostatics.h:38-60 without the FP32 rounding).
+ *   o64_field_jvp  (dE/dx) w = s * sum_i q_i (w/r^3 - 3 d (d.w)/r^5), the
+ *                  field-gradient tensor applied to w (closed form of the
+ *                  derivative of the expression above).
+ *   o64_trace      the plain (CPU_Implement.cpp:38-102) and angle step rules
+ *                  in double with the physics-form field.
+ *   o64_adjoint    reverse-mode derivative of sum_s G_s . x_s through
+ *                  o64_trace, w.r.t. every charge (x, y, z, q) and every seed.
+ *                  No reference counterpart (the reference has no adjoint); it
+ *                  is pinned by central finite differences of o64_trace in
+ *                  tests/test_oracle.py.
+ *
+ * Parity pins: single-charge closed form (|E| = |q|/r^2, radial), linearity
+ * in q, the divergence of J being zero away from charges (trace of the
+ * gradient tensor), and finite differences for the JVP and the adjoint.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { O64_PLAIN = 0, O64_REF = 1, O64_ANGLE = 2 };
+
+typedef struct o64_step_params {
+    int rule;
+    double resolution, h0, hmin, hmax, turn;
+} o64_step_params;
+
+static inline void neu_add(double* s, double* c, double v)
+{
+    double t = *s + v;
+    if (fabs(*s) >= fabs(v))
+        *c += (*s - t) + v;
+    else
+        *c += (v - t) + *s;
+    *s = t;
+}
+
+static void field64(const double* ch, size_t p, double scale, const double* x,
+                    double* E)
+{
+    double s[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    for (size_t i = 0; i < p; ++i) {
+        const double* q = ch + 4 * i;
+        double d[3] = {x[0] - q[0], x[1] - q[1], x[2] - q[2]};
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double i3 = q[3] / (r2 * sqrt(r2));
+        for (int k = 0; k < 3; ++k) neu_add(&s[k], &c[k], i3 * d[k]);
+    }
+    for (int k = 0; k < 3; ++k) E[k] = scale * (s[k] + c[k]);
+}
+
+static void jvp64(const double* ch, size_t p, double scale, const double* x,
+                  const double* w, double* out)
+{
+    double s[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    for (size_t i = 0; i < p; ++i) {
+        const double* q = ch + 4 * i;
+        double d[3] = {x[0] - q[0], x[1] - q[1], x[2] - q[2]};
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double r = sqrt(r2);
+        double i3 = 1.0 / (r2 * r), i5 = i3 / r2;
+        double a = d[0] * w[0] + d[1] * w[1] + d[2] * w[2];
+        for (int k = 0; k < 3; ++k)
+            neu_add(&s[k], &c[k], q[3] * (w[k] * i3 - 3.0 * d[k] * a * i5));
+    }
+    for (int k = 0; k < 3; ++k) out[k] = scale * (s[k] + c[k]);
+}
+
+void o64_field(const double* ch, size_t p, const double* pts, size_t m,
+               double scale, double* E)
+{
+    long long ml = (long long)m;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < ml; ++i) field64(ch, p, scale, pts + 3 * i, E + 3 * i);
+}
+
+void o64_field_jvp(const double* ch, size_t p, const double* pts, size_t m,
+                   double scale, const double* w, double* out)
+{
+    long long ml = (long long)m;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < ml; ++i)
+        jvp64(ch, p, scale, pts + 3 * i, w + 3 * i, out + 3 * i);
+}
+
+/* One step of the rule.  Returns h; writes u and the rule's branch flag
+ * (0: h from the turn ratio, 1: clamped to hmax, 2: clamped to hmin,
+ *  3: constant step). */
+static double step_rule(const o64_step_params* sp, int step, const double* E,
+                        const double* up, double hprev, double* u,
+                        double* L, double* c, int* branch)
+{
+    double l = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+    *L = l;
+    for (int k = 0; k < 3; ++k) u[k] = E[k] / l;
+    *c = 0;
+    if (sp->rule != O64_ANGLE) {
+        *branch = 3;
+        return 1.0 / sp->resolution;
+    }
+    if (step == 1) {
+        *branch = 3;
+        return sp->h0;
+    }
+    double d[3] = {u[0] - up[0], u[1] - up[1], u[2] - up[2]};
+    double cc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    *c = cc;
+    double th = sp->turn * hprev, h;
+    if (cc * sp->hmax > th) {
+        h = th / cc;
+        *branch = 0;
+    } else {
+        h = sp->hmax;
+        *branch = 1;
+    }
+    if (!(h > sp->hmin)) {
+        h = sp->hmin;
+        *branch = 2;
+    }
+    return h;
+}
+
+/* rows: len*n*3, step-major, rows[(s*n + line)*3 + k]; row 0 = seeds. */
+int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
+              size_t len, const o64_step_params* sp, double scale, double* rows)
+{
+    if (!n) return 1;
+    if (sp->rule == O64_REF) return 4; /* the REF rule is restated in restate.c */
+    if (sp->rule == O64_PLAIN && sp->resolution == 0) return 2;
+    if (len < 2) return 3;
+    long long nl = (long long)n;
+#pragma omp parallel for schedule(static)
+    for (long long line = 0; line < nl; ++line) {
+        double x[3] = {seeds[3 * line], seeds[3 * line + 1], seeds[3 * line + 2]};
+        double up[3] = {0, 0, 0}, hprev = sp->h0;
+        for (int k = 0; k < 3; ++k) rows[3 * line + k] = x[k];
+        for (size_t s = 1; s < len; ++s) {
+            double E[3], u[3], L, c;
+            int br;
+            field64(ch, p, scale, x, E);
+            double h = step_rule(sp, (int)s, E, up, hprev, u, &L, &c, &br);
+            for (int k = 0; k < 3; ++k) {
+                x[k] = x[k] + h * u[k];
+                rows[(s * n + line) * 3 + k] = x[k];
+                up[k] = u[k];
+            }
+            hprev = h;
+        }
+    }
+    return 0;
+}
+
+/* Reverse-mode derivative of  L = sum_{s=0}^{len-1} G_s . x_s  (G: len*n*3,
+ * same layout as rows) w.r.t. charges (gch: p*4 = d/dx, d/dy, d/dz, d/dq) and
+ * seeds (gseed: n*3). */
+int o64_adjoint(const double* ch, size_t p, const double* seeds, size_t n,
+                size_t len, const o64_step_params* sp, double scale,
+                const double* G, double* gch, double* gseed)
+{
+    if (!n) return 1;
+    if (sp->rule == O64_REF) return 4;
+    if (sp->rule == O64_PLAIN && sp->resolution == 0) return 2;
+    if (len < 2) return 3;
+    memset(gch, 0, sizeof(double) * 4 * p);
+    long long nl = (long long)n;
+#pragma omp parallel
+    {
+        double* acc = (double*)calloc(4 * p, sizeof(double));
+        /* per-step forward record */
+        double* X = (double*)malloc(sizeof(double) * 3 * len); /* x_{s-1} */
+        double* U = (double*)malloc(sizeof(double) * 3 * len);
+        double* Lr = (double*)malloc(sizeof(double) * len);
+        double* Hs = (double*)malloc(sizeof(double) * len);
+        double* Cs = (double*)malloc(sizeof(double) * len);
+        int* Br = (int*)malloc(sizeof(int) * len);
+#pragma omp for schedule(static)
+        for (long long line = 0; line < nl; ++line) {
+            double x[3] = {seeds[3 * line], seeds[3 * line + 1], seeds[3 * line + 2]};
+            double up[3] = {0, 0, 0}, hprev = sp->h0;
+            for (size_t s = 1; s < len; ++s) {
+                double E[3], u[3];
+                for (int k = 0; k < 3; ++k) X[3 * s + k] = x[k];
+                field64(ch, p, scale, x, E);
+                double h = step_rule(sp, (int)s, E, up, hprev, u, &Lr[s], &Cs[s], &Br[s]);
+                Hs[s] = h;
+                for (int k = 0; k < 3; ++k) {
+                    U[3 * s + k] = u[k];
+                    x[k] = x[k] + h * u[k];
+                    up[k] = u[k];
+                }
+                hprev = h;
+            }
+            double Xb[3] = {0, 0, 0}, Ub[3] = {0, 0, 0}, Hb = 0;
+            for (size_t s = len - 1; s >= 1; --s) {
+                const double* u = U + 3 * s;
+                for (int k = 0; k < 3; ++k) Xb[k] += G[(s * n + line) * 3 + k];
+                Hb += Xb[0] * u[0] + Xb[1] * u[1] + Xb[2] * u[2];
+                for (int k = 0; k < 3; ++k) Ub[k] += Hs[s] * Xb[k];
+                double Upb[3] = {0, 0, 0}, hpb = 0;
+                if (sp->rule == O64_ANGLE && s >= 2 && Br[s] == 0) {
+                    const double* u0 = U + 3 * (s - 1);
+                    double cb = -Hb * Hs[s] / Cs[s];
+                    hpb = Hb * sp->turn / Cs[s];
+                    for (int k = 0; k < 3; ++k) {
+                        double dv = (u[k] - u0[k]) / Cs[s];
+                        Ub[k] += cb * dv;
+                        Upb[k] = -cb * dv;
+                    }
+                }
+                double ud = u[0] * Ub[0] + u[1] * Ub[1] + u[2] * Ub[2];
+                double Eb[3];
+                for (int k = 0; k < 3; ++k) Eb[k] = (Ub[k] - u[k] * ud) / Lr[s];
+                const double* xp = X + 3 * s;
+                double jv[3];
+                jvp64(ch, p, scale, xp, Eb, jv);
+                for (int k = 0; k < 3; ++k) Xb[k] += jv[k];
+                /* charge gradients: dE/dq_i = s d/r^3,
+                 * dE/dc_i = -s q_i (I/r^3 - 3 d d^T / r^5) */
+                for (size_t i = 0; i < p; ++i) {
+                    const double* q = ch + 4 * i;
+                    double d[3] = {xp[0] - q[0], xp[1] - q[1], xp[2] - q[2]};
+                    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    double r = sqrt(r2);
+                    double i3 = 1.0 / (r2 * r), i5 = i3 / r2;
+                    double a = d[0] * Eb[0] + d[1] * Eb[1] + d[2] * Eb[2];
+                    acc[4 * i + 3] += scale * a * i3;
+                    for (int k = 0; k < 3; ++k)
+                        acc[4 * i + k] -= scale * q[3] * (Eb[k] * i3 - 3.0 * d[k] * a * i5);
+                }
+                for (int k = 0; k < 3; ++k) Ub[k] = Upb[k];
+                Hb = hpb;
+            }
+            for (int k = 0; k < 3; ++k) gseed[3 * line + k] = Xb[k] + G[3 * line + k];
+        }
+#pragma omp critical
+        for (size_t i = 0; i < 4 * p; ++i) gch[i] += acc[i];
+        free(acc); free(X); free(U); free(Lr); free(Hs); free(Cs); free(Br);
+    }
+    return 0;
+}
+
+/* Distance from each of m points (float32 xyz, stride 3) to the nearest of
+ * p charges: the "stays more than r from any charge" filter of the parity
+ * tests.  Plain brute force in double. */
+void o64_min_dist_f32(const float* ch, size_t p, const float* pts, size_t m, float* out)
+{
+    long long ml = (long long)m;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < ml; ++i) {
+        double best = INFINITY;
+        for (size_t j = 0; j < p; ++j) {
+            double dx = (double)pts[3 * i] - ch[4 * j];
+            double dy = (double)pts[3 * i + 1] - ch[4 * j + 1];
+            double dz = (double)pts[3 * i + 2] - ch[4 * j + 2];
+            double d = dx * dx + dy * dy + dz * dz;
+            if (d < best) best = d;
+        }
+        out[i] = (float)sqrt(best);
+    }
+}
+
+/* Path distance: for every point j < first[l] of line l in `test` (SoA rows
+ * [3][len][n], float32), the distance to the polyline through the points of
+ * the same line in `ref` (same layout), searching segments k in [j-W, j+W].
+ * Measures how far a traced point lies from the reference path, independent
+ * of where along the path it falls.  out: [len][n] (0 beyond first[l]). */
+void o64_path_dist_f32(const float* test, const float* ref, size_t len, size_t n,
+                       const long long* first, long long window, float* out)
+{
+    long long nl = (long long)n;
+    const size_t plane = len * n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long l = 0; l < nl; ++l) {
+        long long fl = first[l] < (long long)len ? first[l] : (long long)len;
+        for (long long j = 0; j < (long long)len; ++j) {
+            if (j >= fl) { out[j * n + l] = 0.f; continue; }
+            double px = test[j * n + l], py = test[plane + j * n + l], pz = test[2 * plane + j * n + l];
+            long long k0 = j - window < 0 ? 0 : j - window;
+            long long k1 = j + window < fl - 1 ? j + window : fl - 1;
+            double best = INFINITY;
+            if (k1 <= k0) {
+                double dx = px - ref[j * n + l], dy = py - ref[plane + j * n + l],
+                       dz = pz - ref[2 * plane + j * n + l];
+                best = dx * dx + dy * dy + dz * dz;
+            }
+            for (long long k = k0; k < k1; ++k) {
+                double ax = ref[k * n + l], ay = ref[plane + k * n + l], az = ref[2 * plane + k * n + l];
+                double bx = ref[(k + 1) * n + l] - ax, by = ref[plane + (k + 1) * n + l] - ay,
+                       bz = ref[2 * plane + (k + 1) * n + l] - az;
+                double wx = px - ax, wy = py - ay, wz = pz - az;
+                double bb = bx * bx + by * by + bz * bz;
+                double t = bb > 0 ? (wx * bx + wy * by + wz * bz) / bb : 0.0;
+                t = t < 0 ? 0 : (t > 1 ? 1 : t);
+                double dx = wx - t * bx, dy = wy - t * by, dz = wz - t * bz;
+                double d = dx * dx + dy * dy + dz * dz;
+                if (d < best) best = d;
+            }
+            out[j * n + l] = (float)sqrt(best);
+        }
+    }
+}

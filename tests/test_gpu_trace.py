@@ -1,0 +1,209 @@
+"""GPU parity tests of the forward tracer (HIP path through the C ABI).
+
+Parity bar (float32 trajectories):
+  * plain / ref rules: GPU points vs the CPU path at the same step index;
+    angle rule: distance of GPU points to the CPU path's polyline (the
+    adaptive step makes along-line position an ill-conditioned function of
+    rounding, see tests/helpers.py);
+  * lines whose CPU float32 path tracks the FP64 path within 2e-5 must lie
+    within 1e-4 of the CPU path; over all lines, the GPU's distance to FP64
+    must be no worse than the CPU float32 path's (median and 90th percentile);
+  * the literal rule -- every point within 1e-4 for lines that stay more than
+    0.05 from any charge -- is reported for every case.
+Field evaluations: within 1e-5 relative of the FP64 field where the field is
+not dominated by cancellation (|E| >= 0.1 * sum_i |E_i|).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from emtrace import gen, rules
+from helpers import parity_report, rows64_to_soa
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def gpu_trace(em, w_charges, seeds, length, rule, launch=None):
+    rows = em.trace(dev(w_charges), dev(seeds), length, rule, launch)
+    torch.cuda.synchronize()
+    return rows.cpu().numpy()
+
+
+def print_report(tag, rep):
+    keys = ["lines", "eligible_lines", "eligible_within_tol", "prefix_within_tol",
+            "prefix_median_err", "prefix_max_err", "path_within_tol", "path_max_err"]
+    print(f"[parity] {tag}: " + ", ".join(f"{k}={rep[k]:.3g}" if isinstance(rep[k], float)
+                                         else f"{k}={rep[k]}" for k in keys), flush=True)
+
+
+def check_against_cpu(tag, g_rows, c_rows, r64, charges, rule, tol=1e-4):
+    rep, (perr_g, path_g), first, eligible = parity_report(g_rows, c_rows, charges, tol)
+    print_report(tag + " gpu-vs-cpu", rep)
+    rep_c, (perr_c, path_c), _, _ = parity_report(c_rows, r64, charges, tol)
+    rep_g, (perr_g64, path_g64), _, _ = parity_report(g_rows, r64, charges, tol)
+    print_report(tag + " cpu-vs-fp64", rep_c)
+    print_report(tag + " gpu-vs-fp64", rep_g)
+    use_path = rule.rule == rules.ANGLE
+    e_gc = path_g if use_path else perr_g
+    e_c64 = path_c if use_path else perr_c
+    e_g64 = path_g64 if use_path else perr_g64
+    conditioned = e_c64 <= 2e-5
+    print(f"[parity] {tag}: conditioned lines {conditioned.sum()}/{len(conditioned)}; "
+          f"max gpu-vs-cpu on them {e_gc[conditioned].max() if conditioned.any() else 0:.3g}", flush=True)
+    assert conditioned.mean() > 0.5
+    assert (e_gc[conditioned] <= tol).mean() >= 0.99
+    assert np.median(e_g64) <= 2.0 * np.median(e_c64) + 1e-6
+    assert np.quantile(e_g64, 0.9) <= 2.0 * np.quantile(e_c64, 0.9) + 1e-6
+    return rep
+
+
+def _golden_f32():
+    return sorted(p for p in os.listdir(GOLDEN) if p.startswith("ref_f32"))
+
+
+@pytest.mark.parametrize("name", _golden_f32())
+def test_gpu_matches_reference_golden(gpu, name):
+    """Reference CalcField_CPU<float> output (compiled reference) vs the HIP path."""
+    g = np.load(os.path.join(GOLDEN, name))
+    res = float(g["resolution"])
+    rule = rules.ref(res) if int(g["use_curvature"]) else rules.plain(res)
+    rows = gpu_trace(gpu, g["charges"], g["seeds"], g["rows"].shape[1], rule)
+    assert np.array_equal(rows[:, 0], g["rows"][:, 0])
+    assert np.isfinite(rows).all()
+    step = 1.0 / res
+    rep, (perr, _), first, _ = parity_report(rows, g["rows"], g["charges"], tol=1e-4 * step / 0.1)
+    print_report(name, rep)
+    assert rep["prefix_median_err"] <= 1e-5 * step / 0.1
+    assert rep["prefix_within_tol"] >= 0.95
+
+
+@pytest.mark.parametrize("rule", [rules.ref(10.0), rules.plain(10.0), rules.angle()],
+                         ids=["ref", "plain", "angle"])
+def test_config0_lines_vs_cpu_path(gpu, orc, rule):
+    """Config 0 charges (1,000 in [-10,10]^3), 512 of its 8,192 lines, 2,500 rows."""
+    w = gen.config0()
+    idx = np.arange(0, w.n, 16)
+    seeds = w.seeds[idx]
+    g_rows = gpu_trace(gpu, w.charges, seeds, w.len, rule)
+    rc, c_rows = orc.calc_lines(w.charges, seeds, w.len, rule)
+    assert rc == 0
+    r64_rule = rules.plain(rule.resolution) if rule.rule == rules.REF else rule
+    r64 = rows64_to_soa(orc.trace64(w.charges, seeds, w.len, r64_rule))
+    assert np.isfinite(g_rows).all()
+    check_against_cpu(f"config0/{rules.NAMES[rule.rule]}", g_rows, c_rows, r64, w.charges, rule)
+
+
+def test_field_matches_fp64(gpu, orc):
+    """Per-evaluation field vs FP64: <= 1e-5 relative where not cancellation-dominated."""
+    w = gen.config0()
+    rng = np.random.default_rng(3)
+    pts = np.concatenate([rng.uniform(-12, 12, (4000, 3)),
+                          w.seeds[:2000].astype(np.float64)]).astype(np.float32)
+    for scale in (1.0, rules.K_E):
+        E = gpu.field(dev(w.charges), dev(pts), scale).cpu().numpy().astype(np.float64)
+        E64 = orc.field64(w.charges.astype(np.float64), pts.astype(np.float64), scale)
+        d = pts.astype(np.float64)[:, None, :] - w.charges[None, :, :3].astype(np.float64)
+        r = np.linalg.norm(d, axis=2)
+        mag = (np.abs(w.charges[None, :, 3]) / r ** 2).sum(axis=1) * scale
+        n64 = np.linalg.norm(E64, axis=1)
+        rel = np.linalg.norm(E - E64, axis=1) / n64
+        ok = n64 >= 0.1 * mag
+        print(f"[field] scale={scale:.3g}: {ok.sum()} well-conditioned points, max rel "
+              f"{rel[ok].max():.3g}, median {np.median(rel[ok]):.3g}; all points max "
+              f"|dE|/sum|E_i| {(np.linalg.norm(E - E64, axis=1) / mag).max():.3g}", flush=True)
+        assert ok.sum() > 1000
+        assert rel[ok].max() <= 1e-5
+        assert (np.linalg.norm(E - E64, axis=1) / mag).max() <= 1e-6
+
+
+def test_single_charge_radial_lines(gpu):
+    c = np.array([[1.25, -0.5, 2.0, 0.8]], np.float32)
+    d = gen.fibonacci_dirs(256)
+    seeds = (c[0, :3].astype(np.float64) + 0.1 * d).astype(np.float32)
+    rows = gpu_trace(gpu, c, seeds, 60, rules.plain(10.0))
+    s = np.arange(60)[:, None, None]
+    expect = c[0, :3][None, None, :] + (0.1 + s * 0.1) * d[None]
+    assert np.abs(rows.transpose(1, 2, 0) - expect).max() < 2e-5
+    rows = gpu_trace(gpu, c, seeds, 30, rules.angle())
+    r = np.array([0.1, 0.2] + [0.2 + 0.5 * k for k in range(1, 29)])
+    expect = c[0, :3][None, None, :] + r[:, None, None] * d[None]
+    assert np.abs(rows.transpose(1, 2, 0) - expect).max() < 5e-5
+
+
+def test_ref_rule_bitwise_equals_plain(gpu):
+    """(k+1) rounds to 1 with k_e in E, so the reference's curvature step is its plain step."""
+    w = gen.plain_config(500, 1024, 10.0, 300, seed=1)
+    a = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.plain(10.0))
+    b = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.ref(10.0))
+    same = (a == b).all(axis=(0, 1))
+    assert same.mean() > 0.99, same.mean()
+
+
+@pytest.mark.parametrize("S", [1, 2, 4, 8, 16, 32])
+def test_lanes_per_line_agree(gpu, S):
+    w = gen.plain_config(700, 300, 10.0, 200, seed=6)
+    base = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.plain(10.0),
+                     launch=gpu.make_launch(1, 0, 256))
+    rows = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.plain(10.0),
+                     launch=gpu.make_launch(S, 0, 256))
+    rep, *_ = parity_report(rows, base, w.charges)
+    assert rep["prefix_median_err"] < 1e-5 and rep["prefix_within_tol"] > 0.95
+
+
+def test_multi_tile_agrees_with_single_tile(gpu):
+    w = gen.plain_config(3000, 256, 12.0, 120, seed=8)
+    single = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.angle(), launch=gpu.make_launch(2, 0, 256))
+    multi = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.angle(), launch=gpu.make_launch(2, 512, 256))
+    rep, *_ = parity_report(multi, single, w.charges)
+    assert rep["path_within_tol"] > 0.95
+
+
+def test_deterministic_and_scale_invariant(gpu):
+    w = gen.plain_config(800, 512, 10.0, 150, seed=12)
+    a = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.angle())
+    b = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.angle())
+    assert np.array_equal(a, b)
+    q2 = w.charges.copy()
+    q2[:, 3] *= 2.0  # exact in float: every partial sum doubles, directions are bitwise equal
+    c = gpu_trace(gpu, q2, w.seeds, w.len, rules.angle())
+    assert np.array_equal(a, c)
+
+
+def test_drop_in_error_codes(gpu):
+    L = gpu.lib()
+    ch = np.zeros((4, 4), np.float32)
+    x = np.zeros(640, np.float32)
+    y = np.zeros(640, np.float32)
+    z = np.zeros(640, np.float32)
+    P = lambda a: a.ctypes.data  # noqa: E731
+    assert L.em_calc_lines_f32(P(ch), 4, P(x), P(y), P(z), 0, 10, 10.0, 1, None) == 1
+    assert L.em_calc_lines_f32(P(ch), 4, P(x), P(y), P(z), 64, 10, 0.0, 1, None) == 2
+    assert L.em_calc_lines_f32(P(ch), 4, P(x), P(y), P(z), 64, 1, 10.0, 1, None) == 3
+    # unlike the SSE path (status 5), n need not be a multiple of 64
+    w = gen.plain_config(50, 40, 5.0, 16, seed=2)
+    rows = np.zeros((3, w.len, w.n), np.float32)
+    rows[:, 0] = w.seeds.T
+    rc, perf = gpu.calc_lines(w.charges, rows[0], rows[1], rows[2], w.n, 10.0, True)
+    assert rc == 0 and perf["progress"] == 1.0 and perf["kernel_s"] > 0
+    assert np.isfinite(rows).all()
+    assert perf["interactions"] == 50 * 40 * 15
+
+
+def test_drop_in_matches_device_path(gpu):
+    w = gen.plain_config(1000, 640, 10.0, 100, seed=4)
+    rows = np.zeros((3, w.len, w.n), np.float32)
+    rows[:, 0] = w.seeds.T
+    rc, perf = gpu.calc_lines(w.charges, rows[0], rows[1], rows[2], w.n, 10.0, True)
+    assert rc == 0
+    dev_rows = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.ref(10.0))
+    assert np.array_equal(rows, dev_rows)
+    assert perf["h2d_bytes"] > 0 and perf["d2h_bytes"] == 3 * 4 * w.n * (w.len - 1)
