@@ -1,21 +1,618 @@
-// adjoint.hip -- placeholder until the adjoint kernels land.
+// adjoint.hip -- reverse-mode derivative of the tracer (no reference
+// counterpart; the reference only traces forward).
+//
+// Forward model per line (rule = plain or angle), E = f * sum_i q_i d_i/|d_i|^3:
+//   u_s = E(x_{s-1})/|E(x_{s-1})|,  h_s = H(|u_s - u_{s-1}|, h_{s-1}),
+//   x_s = x_{s-1} + h_s u_s.
+// Given G_s = dL/dx_s for every row, three kernels produce dL/d(charges) and
+// dL/d(seeds):
+//   k_adjoint_lines<S>   per line, segment by segment from the last: recompute
+//                        the segment from its checkpoint (bit-identical to the
+//                        forward: same field_at and step_update), then sweep it
+//                        backward through x_s = x_{s-1} + h u, the step rule and
+//                        u = E/|E|, apply the field-gradient JVP
+//                        (dE/dx) w = f sum_i q_i (w/r^3 - 3 d (d.w)/r^5)
+//                        (an N-body pass over the charges), and store
+//                        w_s = f * dL/dE_s for every step;
+//   k_adjoint_charges    charges as targets, the (x_{s-1}, w_s) pairs as
+//                        sources (a second N-body pass): per charge
+//                          dL/dq = sum w.d/r^3,
+//                          dL/dc = -q sum (w/r^3 - 3 d (d.w)/r^5),
+//                        one partial per (charge, source chunk);
+//   k_adjoint_reduce     sums the chunks in a fixed order (deterministic).
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+
 #include "em_internal.hpp"
+
+namespace em {
+
+struct ChargeItem {
+    uint32_t cfg;
+    uint32_t charge0;  // first charge of the block
+    uint32_t src0;     // first source (index into the config's n*(len-1) sources)
+    uint32_t src_cnt;  // sources in this chunk
+    uint64_t part;     // first partial (float4) of this item: part + ci
+};
+
+struct RedItem {
+    uint32_t cfg;
+    uint32_t charge0;
+    uint32_t chunks;   // partials to sum per charge
+    uint32_t pad;
+    uint64_t part;     // partial of (chunk 0, charge 0); chunk stride = p
+};
+
+namespace {
+
+// ---- field-gradient JVP body: (dE/dx) w without the factor f -------------
+struct JvpBody {
+    float px, py, pz, wx, wy, wz;
+    float s3, gx, gy, gz;            // block partials: sum q/r^3, sum a q/r^5 d
+    float S3, cS3, Gx, cGx, Gy, cGy, Gz, cGz;
+    __device__ void init(float x, float y, float z, float a, float b, float c)
+    {
+        px = x; py = y; pz = z; wx = a; wy = b; wz = c;
+        s3 = gx = gy = gz = 0.f;
+        S3 = cS3 = Gx = cGx = Gy = cGy = Gz = cGz = 0.f;
+    }
+    __device__ __forceinline__ void term(const float4* rec)
+    {
+        const float4 c = rec[0];
+        const float dx = px - c.x;
+        const float dy = py - c.y;
+        const float dz = pz - c.z;
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, kSoft2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float i3q = (c.w * inv) * inv2;
+        s3 += i3q;
+        const float a = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
+        const float b = (a * i3q) * inv2;
+        gx = __builtin_fmaf(b, dx, gx);
+        gy = __builtin_fmaf(b, dy, gy);
+        gz = __builtin_fmaf(b, dz, gz);
+    }
+    __device__ __forceinline__ void fold()
+    {
+        em::fold(S3, cS3, s3);
+        em::fold(Gx, cGx, gx);
+        em::fold(Gy, cGy, gy);
+        em::fold(Gz, cGz, gz);
+        s3 = gx = gy = gz = 0.f;
+    }
+};
+
+template <int S>
+__device__ __forceinline__ void jvp_at(float4* __restrict__ tile, uint32_t tile_cap,
+                                       const float4* __restrict__ charges, uint32_t p, int k,
+                                       float px, float py, float pz, float wx, float wy, float wz,
+                                       float& jx, float& jy, float& jz)
+{
+    JvpBody body;
+    body.init(px, py, pz, wx, wy, wz);
+    constexpr uint32_t G = S * kGroup;
+    const uint32_t pp = round_up(p, G);
+    if (pp <= tile_cap) {
+        pipelined_tile<S, 1>(tile, pp, k, body);
+    } else {
+        for (uint32_t t0 = 0; t0 < p; t0 += tile_cap) {
+            const uint32_t cnt = min(tile_cap, p - t0);
+            const uint32_t cntp = round_up(cnt, G);
+            __syncthreads();
+            stage(tile, charges + t0, cnt, cntp);
+            __syncthreads();
+            pipelined_tile<S, 1>(tile, cntp, k, body);
+        }
+    }
+    const float S3 = group_sum<S>(body.S3 + body.cS3);
+    const float Gx = group_sum<S>(body.Gx + body.cGx);
+    const float Gy = group_sum<S>(body.Gy + body.cGy);
+    const float Gz = group_sum<S>(body.Gz + body.cGz);
+    jx = wx * S3 - 3.f * Gx;
+    jy = wy * S3 - 3.f * Gy;
+    jz = wz * S3 - 3.f * Gz;
+}
+
+// Per-lane recompute record: three float4 per step, coalesced across lanes:
+// rec[(j*3 + q) * lanes + lane].
+struct RecPtr {
+    float4* base;
+    uint64_t lanes;
+    uint64_t lane;
+    __device__ float4* at(uint32_t j, int q) const { return base + ((uint64_t)j * 3 + q) * lanes + lane; }
+};
+
+template <int S>
+__global__ void __launch_bounds__(512)
+k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
+                const float4* __restrict__ charges, const float* __restrict__ GX,
+                const float* __restrict__ GY, const float* __restrict__ GZ, StepP sp,
+                uint32_t tile_cap, const float* __restrict__ ckpt, uint32_t every,
+                float4* __restrict__ rec_base, float* __restrict__ WX, float* __restrict__ WY,
+                float* __restrict__ WZ, float* __restrict__ gseed)
+{
+    extern __shared__ float4 tile[];
+    const Item item = items[blockIdx.x];
+    const Cfg cfg = cfgs[item.cfg];
+    const int tid = threadIdx.x;
+    const int k = tid % S;
+    const uint32_t line = item.line0 + tid / S;
+    const bool valid = line < cfg.n;
+    const uint32_t li = valid ? line : cfg.n - 1;
+    const bool writer = valid && k == 0;
+    const float4* ch = charges + cfg.charge_off;
+    const uint32_t p = cfg.p;
+    const uint32_t pp = round_up(p, S * kGroup);
+    if (pp <= tile_cap) {
+        stage(tile, ch, p, pp);
+        __syncthreads();
+    }
+    RecPtr rec{rec_base, (uint64_t)gridDim.x * blockDim.x, (uint64_t)blockIdx.x * blockDim.x + tid};
+    const uint32_t steps = cfg.len - 1;
+    const uint32_t nseg = (steps + every - 1) / every;
+    float lx = 0.f, ly = 0.f, lz = 0.f;     // dL/dx carried backward
+    float lux = 0.f, luy = 0.f, luz = 0.f;  // dL/du_s from step s+1's chord
+    float lh = 0.f;                          // dL/dh_s from step s+1's rule
+    for (int m = (int)nseg - 1; m >= 0; --m) {
+        const uint32_t s0 = (uint32_t)m * every;
+        const uint32_t s1 = min(s0 + every, steps);
+        const float4* ck = reinterpret_cast<const float4*>(ckpt + cfg.ckpt_off) +
+                           2ull * ((uint64_t)m * cfg.n + li);
+        const float4 c0 = ck[0], c1 = ck[1];
+        LineState st;
+        st.px = c0.x; st.py = c0.y; st.pz = c0.z; st.h = c0.w;
+        st.ux = c1.x; st.uy = c1.y; st.uz = c1.z;
+        st.Px = st.Py = st.Pz = 0.f;
+        const float u0x = c1.x, u0y = c1.y, u0z = c1.z;
+        // recompute the segment, keeping what the backward sweep needs
+        for (uint32_t s = s0 + 1; s <= s1; ++s) {
+            float ex, ey, ez;
+            field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
+            ex *= sp.scale;
+            ey *= sp.scale;
+            ez *= sp.scale;
+            const float xx = st.px, xy = st.py, xz = st.pz;
+            StepRec r;
+            step_update(sp, s, ex, ey, ez, st, &r);
+            const uint32_t j = s - s0 - 1;
+            *rec.at(j, 0) = make_float4(xx, xy, xz, r.L);
+            *rec.at(j, 1) = make_float4(r.ux, r.uy, r.uz, r.h);
+            *rec.at(j, 2) = make_float4(r.c, (float)r.branch, 0.f, 0.f);
+        }
+        // backward sweep
+        for (uint32_t s = s1; s > s0; --s) {
+            const uint32_t j = s - s0 - 1;
+            const float4 r0 = *rec.at(j, 0), r1 = *rec.at(j, 1), r2 = *rec.at(j, 2);
+            const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + li;
+            lx += GX[o];
+            ly += GY[o];
+            lz += GZ[o];
+            const float ux = r1.x, uy = r1.y, uz = r1.z, h = r1.w;
+            lh += lx * ux + ly * uy + lz * uz;
+            lux += h * lx;
+            luy += h * ly;
+            luz += h * lz;
+            float lpx = 0.f, lpy = 0.f, lpz = 0.f, hpb = 0.f;
+            if (sp.rule == kRuleAngle && s >= 2 && r2.y == 0.f) {
+                float vx, vy, vz;
+                if (j > 0) {
+                    const float4 q1 = *rec.at(j - 1, 1);
+                    vx = q1.x; vy = q1.y; vz = q1.z;
+                } else {
+                    vx = u0x; vy = u0y; vz = u0z;
+                }
+                const float c = r2.x;
+                const float cb = -lh * h / c;
+                hpb = lh * sp.turn / c;
+                const float dvx = (ux - vx) / c, dvy = (uy - vy) / c, dvz = (uz - vz) / c;
+                lux += cb * dvx;
+                luy += cb * dvy;
+                luz += cb * dvz;
+                lpx = -cb * dvx;
+                lpy = -cb * dvy;
+                lpz = -cb * dvz;
+            }
+            const float ud = ux * lux + uy * luy + uz * luz;
+            const float L = r0.w;
+            const float wx = sp.scale * ((lux - ux * ud) / L);
+            const float wy = sp.scale * ((luy - uy * ud) / L);
+            const float wz = sp.scale * ((luz - uz * ud) / L);
+            if (writer) {
+                WX[o] = wx;
+                WY[o] = wy;
+                WZ[o] = wz;
+            }
+            float jx, jy, jz;
+            jvp_at<S>(tile, tile_cap, ch, p, k, r0.x, r0.y, r0.z, wx, wy, wz, jx, jy, jz);
+            lx += jx;
+            ly += jy;
+            lz += jz;
+            lux = lpx;
+            luy = lpy;
+            luz = lpz;
+            lh = hpb;
+        }
+    }
+    if (writer && gseed) {
+        const uint64_t o = cfg.line_off + li;
+        const uint64_t plane = (uint64_t)cfg.grad_line_off;  // seeds gradient plane stride
+        gseed[line] = lx + GX[o];
+        gseed[plane + line] = ly + GY[o];
+        gseed[2 * plane + line] = lz + GZ[o];
+    }
+}
+
+// ---- charge-gradient body: charge = target, (x, w) = source -----------------
+struct ChargeBody {
+    float cx, cy, cz;
+    float gq, gwx, gwy, gwz, gdx, gdy, gdz;
+    float Gq, cGq, Wx, cWx, Wy, cWy, Wz, cWz, Dx, cDx, Dy, cDy, Dz, cDz;
+    __device__ void init(float x, float y, float z)
+    {
+        cx = x; cy = y; cz = z;
+        gq = gwx = gwy = gwz = gdx = gdy = gdz = 0.f;
+        Gq = cGq = Wx = cWx = Wy = cWy = Wz = cWz = Dx = cDx = Dy = cDy = Dz = cDz = 0.f;
+    }
+    __device__ __forceinline__ void term(const float4* rec)
+    {
+        const float4 x = rec[0], w = rec[1];
+        const float dx = x.x - cx;
+        const float dy = x.y - cy;
+        const float dz = x.z - cz;
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, kSoft2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float inv2 = inv * inv;
+        const float i3 = inv * inv2;
+        const float a = __builtin_fmaf(dz, w.z, __builtin_fmaf(dy, w.y, dx * w.x));
+        gq = __builtin_fmaf(a, i3, gq);
+        gwx = __builtin_fmaf(w.x, i3, gwx);
+        gwy = __builtin_fmaf(w.y, i3, gwy);
+        gwz = __builtin_fmaf(w.z, i3, gwz);
+        const float b = (a * i3) * inv2;
+        gdx = __builtin_fmaf(b, dx, gdx);
+        gdy = __builtin_fmaf(b, dy, gdy);
+        gdz = __builtin_fmaf(b, dz, gdz);
+    }
+    __device__ __forceinline__ void fold()
+    {
+        em::fold(Gq, cGq, gq);
+        em::fold(Wx, cWx, gwx);
+        em::fold(Wy, cWy, gwy);
+        em::fold(Wz, cWz, gwz);
+        em::fold(Dx, cDx, gdx);
+        em::fold(Dy, cDy, gdy);
+        em::fold(Dz, cDz, gdz);
+        gq = gwx = gwy = gwz = gdx = gdy = gdz = 0.f;
+    }
+};
+
+constexpr uint32_t kSrcTile = 2048;  // sources per LDS tile (2 float4 each: 64 KiB)
+
+__global__ void __launch_bounds__(256)
+k_adjoint_charges(const Cfg* __restrict__ cfgs, const ChargeItem* __restrict__ items,
+                  const float4* __restrict__ charges, const float* __restrict__ X,
+                  const float* __restrict__ Y, const float* __restrict__ Z,
+                  const float* __restrict__ WX, const float* __restrict__ WY,
+                  const float* __restrict__ WZ, float4* __restrict__ part)
+{
+    extern __shared__ float4 tile[];
+    const ChargeItem it = items[blockIdx.x];
+    const Cfg cfg = cfgs[it.cfg];
+    const uint32_t ci = it.charge0 + threadIdx.x;
+    const bool valid = ci < cfg.p;
+    const float4 c = charges[cfg.charge_off + (valid ? ci : 0)];
+    ChargeBody body;
+    body.init(c.x, c.y, c.z);
+    const uint32_t n = cfg.n;
+    for (uint32_t t0 = 0; t0 < it.src_cnt; t0 += kSrcTile) {
+        const uint32_t cnt = min(kSrcTile, it.src_cnt - t0);
+        const uint32_t cntp = round_up(cnt, kGroup);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < cntp; j += blockDim.x) {
+            float4 xr = make_float4(kFar, kFar, kFar, 0.f), wr = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j < cnt) {
+                const uint32_t g = it.src0 + t0 + j;  // source index: (s-1)*n + line
+                const uint32_t s = g / n + 1;
+                const uint32_t line = g - (s - 1) * n;
+                const uint64_t ox = cfg.line_off + (uint64_t)(s - 1) * cfg.pitch + line;
+                const uint64_t ow = cfg.line_off + (uint64_t)s * cfg.pitch + line;
+                xr = make_float4(X[ox], Y[ox], Z[ox], 0.f);
+                wr = make_float4(WX[ow], WY[ow], WZ[ow], 0.f);
+            }
+            tile[2 * j] = xr;
+            tile[2 * j + 1] = wr;
+        }
+        __syncthreads();
+        pipelined_tile<1, 2>(tile, cntp, 0, body);
+    }
+    if (valid) {
+        const float Hx = (body.Wx + body.cWx) - 3.f * (body.Dx + body.cDx);
+        const float Hy = (body.Wy + body.cWy) - 3.f * (body.Dy + body.cDy);
+        const float Hz = (body.Wz + body.cWz) - 3.f * (body.Dz + body.cDz);
+        part[it.part + threadIdx.x] = make_float4(Hx, Hy, Hz, body.Gq + body.cGq);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_adjoint_reduce(const Cfg* __restrict__ cfgs, const RedItem* __restrict__ items,
+                 const float4* __restrict__ charges, const float4* __restrict__ part,
+                 float4* __restrict__ grad)
+{
+    const RedItem it = items[blockIdx.x];
+    const Cfg cfg = cfgs[it.cfg];
+    const uint32_t ci = it.charge0 + threadIdx.x;
+    if (ci >= cfg.p) return;
+    float sx = 0.f, sy = 0.f, sz = 0.f, sq = 0.f, ex = 0.f, ey = 0.f, ez = 0.f, eq = 0.f;
+    for (uint32_t c = 0; c < it.chunks; ++c) {
+        const float4 v = part[it.part + (uint64_t)c * cfg.p + threadIdx.x];
+        fold(sx, ex, v.x);
+        fold(sy, ey, v.y);
+        fold(sz, ez, v.z);
+        fold(sq, eq, v.w);
+    }
+    const float q = charges[cfg.charge_off + ci].w;
+    grad[cfg.charge_off + ci] = make_float4(-q * (sx + ex), -q * (sy + ey), -q * (sz + ez), sq + eq);
+}
+
+template <int S>
+struct LinesL {
+    static int run(const Shape& sh, uint32_t n_items, const Cfg* cfgs, const Item* items,
+                   const float4* charges, const float* gx, const float* gy, const float* gz,
+                   const StepP& sp, const float* ckpt, uint32_t every, float4* rec, float* wx,
+                   float* wy, float* wz, float* gseed, hipStream_t stream)
+    {
+        const size_t lds = (size_t)sh.tile * sizeof(float4);
+        if (lds > 65536)
+            EM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_adjoint_lines<S>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_adjoint_lines<S>, dim3(n_items), dim3(sh.block), lds, stream, cfgs,
+                           items, charges, gx, gy, gz, sp, sh.tile, ckpt, every, rec, wx, wy, wz,
+                           gseed);
+        EM_HIP(hipGetLastError());
+        return EM_OK;
+    }
+};
+
+template <template <int> class F, typename... A>
+int dispatch_S(int S, A&&... a)
+{
+    switch (S) {
+        case 1: return F<1>::run(a...);
+        case 2: return F<2>::run(a...);
+        case 4: return F<4>::run(a...);
+        case 8: return F<8>::run(a...);
+        case 16: return F<16>::run(a...);
+        case 32: return F<32>::run(a...);
+        default: EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1,2,4,8,16,32}", S);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side: work lists and buffers of one adjoint configuration set.
+struct AdjPlan {
+    int device = -1;
+    Shape sh;
+    uint32_t every = 0;
+    std::vector<Item> line_items;
+    std::vector<ChargeItem> charge_items;
+    std::vector<RedItem> red_items;
+    Item* d_line_items = nullptr;
+    ChargeItem* d_charge_items = nullptr;
+    RedItem* d_red_items = nullptr;
+    Cfg* d_cfgs_own = nullptr;  // single-configuration calls upload their own descriptor
+    float4* d_rec = nullptr;
+    size_t rec_bytes = 0;
+    float* d_w = nullptr;       // 3 planes of w (rows layout)
+    size_t w_plane = 0;
+    float4* d_part = nullptr;
+    size_t part_bytes = 0;
+};
+
+static int grow(void** p, size_t* have, size_t need)
+{
+    if (*p && *have >= need) return EM_OK;
+    if (*p) EM_HIP(hipFree(*p));
+    *p = nullptr;
+    *have = 0;
+    EM_HIP(hipMalloc(p, std::max<size_t>(need, 256)));
+    *have = std::max<size_t>(need, 256);
+    return EM_OK;
+}
+
+int adj_free(AdjPlan* a)
+{
+    if (!a) return EM_OK;
+    void* ps[] = {a->d_line_items, a->d_charge_items, a->d_red_items, a->d_cfgs_own,
+                  a->d_rec, a->d_w, a->d_part};
+    for (void* p : ps)
+        if (p) EM_HIP(hipFree(p));
+    *a = AdjPlan();
+    return EM_OK;
+}
+
+// Build the work lists for `cfgs` (host copies; ckpt_off already set).
+int adj_prepare(AdjPlan* a, const std::vector<Cfg>& cfgs, const Shape& sh, uint32_t every,
+                const DeviceInfo& dev)
+{
+    a->device = dev.device;
+    a->sh = sh;
+    a->every = every;
+    a->line_items.clear();
+    a->charge_items.clear();
+    a->red_items.clear();
+    const uint32_t lpb = (uint32_t)(sh.block / sh.S);
+    uint64_t line_elems = 0;
+    double total = 0.0;
+    for (size_t i = 0; i < cfgs.size(); ++i) {
+        const Cfg& c = cfgs[i];
+        for (uint32_t l0 = 0; l0 < c.n; l0 += lpb) a->line_items.push_back(Item{(uint32_t)i, l0});
+        line_elems = std::max<uint64_t>(line_elems, c.line_off + (uint64_t)c.len * c.pitch);
+        total += (double)c.p * c.n * (c.len - 1);
+    }
+    // charge pass: chunks sized so the whole pass has ~4 items per CU
+    const double per_item = std::max(1.0, total / (4.0 * dev.cus));
+    uint64_t part = 0;
+    for (size_t i = 0; i < cfgs.size(); ++i) {
+        const Cfg& c = cfgs[i];
+        const uint64_t src = (uint64_t)c.n * (c.len - 1);
+        if (!c.p || !src) continue;
+        double want = per_item / std::min<double>(c.p, 256.0);
+        uint64_t chunk = (uint64_t)std::min<double>(std::max(2048.0, want), 4194304.0);
+        chunk = (chunk + 2047) / 2048 * 2048;
+        const uint32_t chunks = (uint32_t)((src + chunk - 1) / chunk);
+        for (uint32_t q0 = 0; q0 < c.p; q0 += 256) {
+            for (uint32_t k = 0; k < chunks; ++k) {
+                ChargeItem it;
+                it.cfg = (uint32_t)i;
+                it.charge0 = q0;
+                it.src0 = (uint32_t)(k * chunk);
+                it.src_cnt = (uint32_t)std::min<uint64_t>(chunk, src - k * chunk);
+                it.part = part + (uint64_t)k * c.p + q0;
+                a->charge_items.push_back(it);
+            }
+            a->red_items.push_back(RedItem{(uint32_t)i, q0, chunks, 0, part + q0});
+        }
+        part += (uint64_t)chunks * c.p;
+    }
+    a->w_plane = line_elems;
+    const uint64_t lanes = (uint64_t)a->line_items.size() * sh.block;
+    EM_TRY(grow(reinterpret_cast<void**>(&a->d_rec), &a->rec_bytes,
+                lanes * (size_t)every * 3 * sizeof(float4)));
+    size_t wb = 0;
+    if (a->d_w) wb = 0;  // always re-size below
+    if (a->d_w) EM_HIP(hipFree(a->d_w));
+    a->d_w = nullptr;
+    EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_w), std::max<size_t>(3 * line_elems * sizeof(float), 256)));
+    (void)wb;
+    EM_TRY(grow(reinterpret_cast<void**>(&a->d_part), &a->part_bytes, part * sizeof(float4)));
+    if (a->d_line_items) EM_HIP(hipFree(a->d_line_items));
+    if (a->d_charge_items) EM_HIP(hipFree(a->d_charge_items));
+    if (a->d_red_items) EM_HIP(hipFree(a->d_red_items));
+    a->d_line_items = nullptr;
+    a->d_charge_items = nullptr;
+    a->d_red_items = nullptr;
+    EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_line_items),
+                     std::max<size_t>(1, a->line_items.size()) * sizeof(Item)));
+    EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_charge_items),
+                     std::max<size_t>(1, a->charge_items.size()) * sizeof(ChargeItem)));
+    EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_red_items),
+                     std::max<size_t>(1, a->red_items.size()) * sizeof(RedItem)));
+    EM_HIP(hipMemcpy(a->d_line_items, a->line_items.data(), a->line_items.size() * sizeof(Item),
+                     hipMemcpyHostToDevice));
+    if (!a->charge_items.empty())
+        EM_HIP(hipMemcpy(a->d_charge_items, a->charge_items.data(),
+                         a->charge_items.size() * sizeof(ChargeItem), hipMemcpyHostToDevice));
+    if (!a->red_items.empty())
+        EM_HIP(hipMemcpy(a->d_red_items, a->red_items.data(), a->red_items.size() * sizeof(RedItem),
+                         hipMemcpyHostToDevice));
+    return EM_OK;
+}
+
+int adj_launch(AdjPlan* a, const Cfg* d_cfgs, const float4* charges, const float* x,
+               const float* y, const float* z, const float* ckpt, const float* gx,
+               const float* gy, const float* gz, const StepP& sp, float4* grad, float* gseed,
+               hipStream_t stream)
+{
+    float* wx = a->d_w;
+    float* wy = wx + a->w_plane;
+    float* wz = wy + a->w_plane;
+    EM_TRY(dispatch_S<LinesL>(a->sh.S, a->sh, (uint32_t)a->line_items.size(), d_cfgs,
+                              a->d_line_items, charges, gx, gy, gz, sp, ckpt, a->every, a->d_rec,
+                              wx, wy, wz, gseed, stream));
+    if (!a->charge_items.empty()) {
+        const size_t lds = (size_t)kSrcTile * 2 * sizeof(float4);
+        hipLaunchKernelGGL(k_adjoint_charges, dim3((uint32_t)a->charge_items.size()), dim3(256), lds,
+                           stream, d_cfgs, a->d_charge_items, charges, x, y, z, wx, wy, wz,
+                           a->d_part);
+        EM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_adjoint_reduce, dim3((uint32_t)a->red_items.size()), dim3(256), 0,
+                           stream, d_cfgs, a->d_red_items, charges, a->d_part, grad);
+        EM_HIP(hipGetLastError());
+    }
+    return EM_OK;
+}
+
+// single-configuration adjoint state, one per device
+static std::mutex g_adj_mu;
+static AdjPlan g_adj[64];
+
+int adj_release_all()
+{
+    std::lock_guard<std::mutex> lk(g_adj_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return EM_OK;
+    for (int i = 0; i < 64; ++i) {
+        if (g_adj[i].device < 0) continue;
+        EM_HIP(hipSetDevice(i));
+        EM_TRY(adj_free(&g_adj[i]));
+    }
+    EM_HIP(hipSetDevice(cur));
+    return EM_OK;
+}
+
+}  // namespace em
 
 using namespace em;
 
-extern "C" {
-
-int em_adjoint_dev_f32(const float*, size_t, const float*, const float*, const float*, size_t,
-                       size_t, size_t, const em_step*, const em_launch*, const float*, uint32_t,
-                       const float*, const float*, const float*, float*, float*, void*)
+extern "C" int em_adjoint_dev_f32(const float* d_charges, size_t p, const float* d_x,
+                                  const float* d_y, const float* d_z, size_t n, size_t len,
+                                  size_t pitch, const em_step* step, const em_launch* launch,
+                                  const float* d_ckpt, uint32_t ckpt_every, const float* d_gx,
+                                  const float* d_gy, const float* d_gz, float* d_grad_charges,
+                                  float* d_grad_seeds, void* stream)
 {
-    EM_FAIL(EM_ERR_UNSUPPORTED, "adjoint not built yet");
+    clear_error();
+    if (!n) EM_FAIL(EM_ERR_NO_LINES, "n == 0");
+    EM_TRY(validate_step(step));
+    if (len < 2) EM_FAIL(EM_ERR_LENGTH, "len %zu < 2", len);
+    if (step->rule == kRuleRef)
+        EM_FAIL(EM_ERR_UNSUPPORTED, "the adjoint supports the plain and angle rules");
+    if (p > 0xFFFFFFFFull || n > 0xFFFFFFFFull || len > 0xFFFFFFFFull || (uint64_t)n * (len - 1) > 0xFFFFFFFFull)
+        EM_FAIL(EM_ERR_INVALID, "sizes out of range");
+    if ((p && (!d_charges || !d_grad_charges)) || !d_x || !d_y || !d_z || !d_ckpt || !d_gx ||
+        !d_gy || !d_gz)
+        EM_FAIL(EM_ERR_INVALID, "NULL device pointer");
+    if (!ckpt_every) EM_FAIL(EM_ERR_INVALID, "ckpt_every == 0");
+    if (pitch < n) EM_FAIL(EM_ERR_INVALID, "pitch < n");
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+    Shape sh;
+    EM_TRY(choose_shape(dev, p, n, launch, sh));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_adj_mu);
+    AdjPlan* a = &g_adj[dev.device];
+    // earlier calls on this stream may still use the shared work buffers
+    EM_HIP(hipStreamSynchronize(st));
+    Cfg cfg{};
+    cfg.pitch = pitch;
+    cfg.p = (uint32_t)p;
+    cfg.n = (uint32_t)n;
+    cfg.len = (uint32_t)len;
+    cfg.grad_line_off = (uint32_t)n;  // seed-gradient plane stride
+    std::vector<Cfg> cfgs{cfg};
+    EM_TRY(adj_prepare(a, cfgs, sh, ckpt_every, dev));
+    if (!a->d_cfgs_own) EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_cfgs_own), sizeof(Cfg)));
+    EM_HIP(hipMemcpy(a->d_cfgs_own, &cfg, sizeof(Cfg), hipMemcpyHostToDevice));
+    if (!p) {
+        if (d_grad_seeds) EM_HIP(hipMemsetAsync(d_grad_seeds, 0, 3 * n * sizeof(float), st));
+        return EM_OK;
+    }
+    return adj_launch(a, a->d_cfgs_own, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
+                      d_ckpt, d_gx, d_gy, d_gz, to_stepp(*step),
+                      reinterpret_cast<float4*>(d_grad_charges), d_grad_seeds, st);
 }
 
-int em_batch_adjoint(em_batch*, const float*, const float*, const float*, const float*,
-                     const float*, uint32_t, const float*, const float*, const float*, float*, void*)
+namespace em {
+AdjPlan* adj_new() { return new (std::nothrow) AdjPlan(); }
+void adj_delete(AdjPlan* a)
 {
-    EM_FAIL(EM_ERR_UNSUPPORTED, "adjoint not built yet");
+    if (!a) return;
+    adj_free(a);
+    delete a;
 }
-
-}
+}  // namespace em

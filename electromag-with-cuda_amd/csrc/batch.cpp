@@ -38,12 +38,16 @@ struct em_batch {
     hipGraphExec_t exec = nullptr;
     const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t graph_every = 0;
+    em::AdjPlan* adj = nullptr;  // adjoint work lists, built on the first adjoint call
+    uint32_t adj_every = 0;
 };
 
 namespace em {
 
 static int free_plan(em_batch* b)
 {
+    if (b->adj) adj_delete(b->adj);
+    b->adj = nullptr;
     if (b->exec) EM_HIP(hipGraphExecDestroy(b->exec));
     if (b->graph) EM_HIP(hipGraphDestroy(b->graph));
     if (b->d_cfgs) EM_HIP(hipFree(b->d_cfgs));
@@ -257,6 +261,39 @@ size_t em_batch_ckpt_floats(const em_batch* plan, uint32_t ckpt_every)
     size_t total = 0;
     for (const auto& c : plan->cfgs) total += 8ull * c.n * ((c.len - 1) / ckpt_every + 1);
     return total;
+}
+
+int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x, const float* d_y,
+                     const float* d_z, const float* d_ckpt, uint32_t ckpt_every, const float* d_gx,
+                     const float* d_gy, const float* d_gz, float* d_grad_charges, void* stream)
+{
+    clear_error();
+    if (!plan || !d_charges || !d_x || !d_y || !d_z || !d_ckpt || !d_gx || !d_gy || !d_gz ||
+        !d_grad_charges)
+        EM_FAIL(EM_ERR_INVALID, "NULL argument");
+    if (!ckpt_every) EM_FAIL(EM_ERR_INVALID, "ckpt_every == 0");
+    if (plan->step.rule == kRuleRef)
+        EM_FAIL(EM_ERR_UNSUPPORTED, "the adjoint supports the plain and angle rules");
+    for (const auto& c : plan->cfgs)
+        if ((uint64_t)c.n * (c.len - 1) > 0xFFFFFFFFull)
+            EM_FAIL(EM_ERR_INVALID, "a configuration has more than 2^32 line-steps");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    EM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device) EM_FAIL(EM_ERR_INVALID, "plan belongs to device %d", plan->device);
+    EM_TRY(set_ckpt_offsets(plan, ckpt_every, st));
+    if (!plan->adj || plan->adj_every != ckpt_every) {
+        EM_HIP(hipStreamSynchronize(st));
+        if (!plan->adj) plan->adj = adj_new();
+        if (!plan->adj) EM_FAIL(EM_ERR_NOMEM, "host allocation failed");
+        DeviceInfo di;
+        EM_TRY(current_device_info(di));
+        EM_TRY(adj_prepare(plan->adj, plan->cfgs, plan->sh, ckpt_every, di));
+        plan->adj_every = ckpt_every;
+    }
+    return adj_launch(plan->adj, plan->d_cfgs, reinterpret_cast<const float4*>(d_charges), d_x, d_y,
+                      d_z, d_ckpt, d_gx, d_gy, d_gz, plan->sp,
+                      reinterpret_cast<float4*>(d_grad_charges), nullptr, st);
 }
 
 int em_batch_run_ckpt(em_batch* plan, const float* d_charges, float* d_x, float* d_y, float* d_z,

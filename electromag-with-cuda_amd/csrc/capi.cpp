@@ -319,7 +319,7 @@ int em_shutdown(void)
         b = HostPathBuffers();
     }
     EM_HIP(hipSetDevice(cur));
-    return EM_OK;
+    return adj_release_all();
 }
 
 int em_calc_lines_f32(const float* charges, size_t p, float* x, float* y, float* z, size_t n,

@@ -56,6 +56,13 @@ struct StepP {
 
 __host__ __device__ constexpr uint32_t round_up(uint32_t a, uint32_t b) { return (a + b - 1) / b * b; }
 
+// r^2 starts from kSoft2 instead of 0 (same instruction count: three FMAs).
+// A point that lands exactly on a charge -- float rounding snaps a line that
+// oscillates about a sink onto the charge's coordinates -- then gets a zero
+// contribution from it instead of 0 * inf = NaN; inv^3 <= 1e30 keeps q * inv^3
+// finite for |q| < 3e8.  Relative effect at r >= 1e-5: below 1e-10.
+constexpr float kSoft2 = 1.0e-20f;
+
 // One charge-point interaction: E += q (p - c) / |p - c|^3.
 __device__ __forceinline__ void coulomb(const float4 c, float px, float py, float pz,
                                         float& ex, float& ey, float& ez)
@@ -63,7 +70,7 @@ __device__ __forceinline__ void coulomb(const float4 c, float px, float py, floa
     const float dx = px - c.x;
     const float dy = py - c.y;
     const float dz = pz - c.z;
-    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, kSoft2)));
     const float inv = __builtin_amdgcn_rsqf(r2);
     const float s = (c.w * inv) * (inv * inv);
     ex = __builtin_fmaf(s, dx, ex);
@@ -86,48 +93,80 @@ struct Acc {
     __device__ void zero() { sx = sy = sz = cx = cy = cz = 0.f; }
 };
 
-// Sum the charges of one LDS tile into acc.  `count` is a multiple of
-// S*kGroup (the tile is padded with q = 0 records far away).  Software-
-// pipelined: while one group of kUnroll charges is being summed, the LDS reads
+// Software-pipelined walk over one LDS tile of records (R float4 each).
+// Lane k of a line group visits records k, k+S, k+2S, ...; `count` records
+// (a multiple of S*kGroup; the tile is padded with records that contribute
+// exactly zero).  While one group of kUnroll records is consumed, the LDS reads
 // of the next group are in flight (two register sets, no copies).
-template <int S>
-__device__ __forceinline__ void sum_tile(const float4* __restrict__ tile, uint32_t count, int k,
-                                         float px, float py, float pz, Acc& acc)
+// Body::term(const float4* rec) consumes one record into plain FP32 partial
+// sums; Body::fold() moves the partials into compensated totals every kBlk
+// records per lane and at the end of the tile.
+template <int S, int R, class Body>
+__device__ __forceinline__ void pipelined_tile(const float4* __restrict__ tile, uint32_t count,
+                                               int k, Body& body)
 {
     const uint32_t ngroups = count / (S * kUnroll);  // even: count is a multiple of S*kGroup
-    const float4* t = tile + k;
-    constexpr uint32_t kStride = S * kUnroll;
+    const float4* t = tile + k * R;
+    constexpr uint32_t kStride = S * kUnroll * R;
     constexpr uint32_t kFoldEvery = kBlk / kUnroll;  // groups per compensated block (even)
-    float4 a[kUnroll], b[kUnroll];
+    float4 a[kUnroll][R], b[kUnroll][R];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) a[u] = t[u * S];
-    float ex = 0.f, ey = 0.f, ez = 0.f;
+    for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[u][r] = t[u * S * R + r];
     for (uint32_t g = 0; g < ngroups; g += 2) {
         // sched_barrier(0) keeps each group's LDS reads issued ahead of the
         // other group's arithmetic (the scheduler would otherwise sink them
         // next to their uses and expose the LDS latency every iteration).
         const float4* tb = t + (g + 1) * kStride;
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) b[u] = tb[u * S];
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) b[u][r] = tb[u * S * R + r];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) coulomb(a[u], px, py, pz, ex, ey, ez);
+        for (int u = 0; u < kUnroll; ++u) body.term(a[u]);
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t gn = min(g + 2, ngroups - 2);  // last pair re-reads itself
         const float4* ta = t + gn * kStride;
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) a[u] = ta[u * S];
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) a[u][r] = ta[u * S * R + r];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) coulomb(b[u], px, py, pz, ex, ey, ez);
+        for (int u = 0; u < kUnroll; ++u) body.term(b[u]);
         __builtin_amdgcn_sched_barrier(0);
-        if ((g + 2) % kFoldEvery == 0 || g + 2 >= ngroups) {
-            fold(acc.sx, acc.cx, ex);
-            fold(acc.sy, acc.cy, ey);
-            fold(acc.sz, acc.cz, ez);
-            ex = ey = ez = 0.f;
-        }
+        if ((g + 2) % kFoldEvery == 0 || g + 2 >= ngroups) body.fold();
     }
+}
+
+// Forward body: E = sum q (p - c)/|p - c|^3 at one point.
+struct FieldBody {
+    float px, py, pz;
+    float ex, ey, ez;  // partials of the current block
+    Acc acc;           // compensated totals
+    __device__ void init(float x, float y, float z)
+    {
+        px = x; py = y; pz = z;
+        ex = ey = ez = 0.f;
+        acc.zero();
+    }
+    __device__ __forceinline__ void term(const float4* c) { coulomb(c[0], px, py, pz, ex, ey, ez); }
+    __device__ __forceinline__ void fold()
+    {
+        em::fold(acc.sx, acc.cx, ex);
+        em::fold(acc.sy, acc.cy, ey);
+        em::fold(acc.sz, acc.cz, ez);
+        ex = ey = ez = 0.f;
+    }
+};
+
+template <int S>
+__device__ __forceinline__ void sum_tile(const float4* __restrict__ tile, uint32_t count, int k,
+                                         FieldBody& body)
+{
+    pipelined_tile<S, 1>(tile, count, k, body);
 }
 
 // Cooperative global -> LDS copy of `cnt` charge records, padded to `cntp`.
@@ -157,12 +196,12 @@ __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t til
                                          int k, float px, float py, float pz,
                                          float& ex, float& ey, float& ez)
 {
-    Acc acc;
-    acc.zero();
+    FieldBody body;
+    body.init(px, py, pz);
     constexpr uint32_t G = S * kGroup;
     const uint32_t pp = round_up(p, G);
     if (pp <= tile_cap) {
-        sum_tile<S>(tile, pp, k, px, py, pz, acc);
+        sum_tile<S>(tile, pp, k, body);
     } else {
         for (uint32_t t0 = 0; t0 < p; t0 += tile_cap) {
             const uint32_t cnt = min(tile_cap, p - t0);
@@ -170,12 +209,12 @@ __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t til
             __syncthreads();
             stage(tile, charges + t0, cnt, cntp);
             __syncthreads();
-            sum_tile<S>(tile, cntp, k, px, py, pz, acc);
+            sum_tile<S>(tile, cntp, k, body);
         }
     }
-    ex = group_sum<S>(acc.sx + acc.cx);
-    ey = group_sum<S>(acc.sy + acc.cy);
-    ez = group_sum<S>(acc.sz + acc.cz);
+    ex = group_sum<S>(body.acc.sx + body.acc.cx);
+    ey = group_sum<S>(body.acc.sy + body.acc.cy);
+    ez = group_sum<S>(body.acc.sz + body.acc.cz);
 }
 
 // Per-line state carried across steps.

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/emtrace.h"
 #include "em_device.hpp"
@@ -75,5 +76,17 @@ int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid);
 int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,
                  const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,
                  float* ez, hipStream_t stream);
+
+// adjoint (adjoint.hip)
+struct AdjPlan;
+AdjPlan* adj_new();
+void adj_delete(AdjPlan* a);
+int adj_prepare(AdjPlan* a, const std::vector<Cfg>& cfgs, const Shape& sh, uint32_t every,
+                const DeviceInfo& dev);
+int adj_launch(AdjPlan* a, const Cfg* d_cfgs, const float4* charges, const float* x,
+               const float* y, const float* z, const float* ckpt, const float* gx,
+               const float* gy, const float* gz, const StepP& sp, float4* grad, float* gseed,
+               hipStream_t stream);
+int adj_release_all();
 
 }  // namespace em

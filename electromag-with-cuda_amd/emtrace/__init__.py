@@ -9,9 +9,11 @@ from ._lib import EmError, LibraryMissing, lib, LIB_PATH
 from .api import (BatchLayout, BatchPlan, calc_lines, ckpt_size, field, make_launch, make_step,
                   pack_batch, seed_rows, trace, trace_host, trace_into)
 from .rules import ANGLE, PLAIN, REF, StepRule
+from .autograd import adjoint, batch_trace, trace_lines
 
 __all__ = [
     "gen", "rules", "EmError", "LibraryMissing", "lib", "LIB_PATH", "BatchLayout", "BatchPlan",
     "calc_lines", "ckpt_size", "field", "make_launch", "make_step", "pack_batch", "seed_rows",
-    "trace", "trace_host", "trace_into", "ANGLE", "PLAIN", "REF", "StepRule",
+    "trace", "trace_host", "trace_into", "ANGLE", "PLAIN", "REF", "StepRule", "adjoint",
+    "batch_trace", "trace_lines",
 ]

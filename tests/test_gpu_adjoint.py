@@ -1,0 +1,153 @@
+"""GPU adjoint vs the FP64 CPU adjoint and FP64 central finite differences.
+
+Loss (differentiation workload): L = mean over points of |x - x_target|^2, the
+targets traced from the same seeds with charges perturbed (positions +
+N(0, sigma^2), magnitudes * U[0.95, 1.05]).  The gradient is checked against
+  * the FP64 adjoint (oracle/oracle64.c: o64_adjoint) with the same dL/dx,
+  * FP64 central differences (h = 1e-4) of the FP64 trace, on sampled
+    parameters,
+with the tolerance written in each test.  Line-step counts are identical by
+construction (every line takes len-1 steps in the forward, the recompute and
+the reverse sweep).
+"""
+import numpy as np
+import pytest
+import torch
+
+from emtrace import gen, rules
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def forward_and_adjoint(em, charges, seeds, length, rule, G, every):
+    """GPU forward (with checkpoints) and adjoint of sum(G . rows)."""
+    c = dev(charges)
+    n = seeds.shape[0]
+    rows = em.seed_rows(dev(seeds), length)
+    ck = torch.empty(max(1, em.ckpt_size(n, length, every)), dtype=torch.float32, device="cuda")
+    em.trace_into(c, rows, n, rule, None, ck, every)
+    gc, gs = em.adjoint(c, rows, ck, every, dev(G.astype(np.float32)), rule)
+    torch.cuda.synchronize()
+    return rows.cpu().numpy(), gc.cpu().numpy().astype(np.float64), gs.cpu().numpy().T.astype(np.float64)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("rule", [rules.plain(10.0), rules.angle()], ids=["plain", "angle"])
+def test_adjoint_matches_fp64_adjoint_small(gpu, orc, rule):
+    w = gen.plain_config(40, 96, 4.0, 60, seed=21)
+    rng = np.random.default_rng(0)
+    G = rng.normal(size=(3, w.len, w.n))
+    rows, gc, gs = forward_and_adjoint(gpu, w.charges, w.seeds, w.len, rule, G, 7)
+    G64 = np.ascontiguousarray(G.transpose(1, 2, 0))
+    gc64, gs64 = orc.adjoint64(w.charges, w.seeds, w.len, rule, G64)
+    print(f"[adjoint small {rules.NAMES[rule.rule]}] rel charges {rel(gc, gc64):.2e} "
+          f"rel seeds {rel(gs, gs64):.2e}", flush=True)
+    assert rel(gc, gc64) < 2e-3
+    assert rel(gs, gs64) < 2e-3
+
+
+def test_checkpoint_interval_does_not_change_result(gpu):
+    """Segment recompute is bit-identical to the forward, so any interval gives the same bits."""
+    w = gen.plain_config(300, 200, 6.0, 90, seed=5)
+    G = np.random.default_rng(1).normal(size=(3, w.len, w.n))
+    ref = None
+    for every in (1, 7, 50, 89, 200):
+        _, gc, gs = forward_and_adjoint(gpu, w.charges, w.seeds, w.len, rules.angle(), G, every)
+        if ref is None:
+            ref = (gc, gs)
+        else:
+            assert np.array_equal(gc, ref[0]) and np.array_equal(gs, ref[1]), every
+
+
+def test_adjoint_multi_tile_matches_single_tile(gpu):
+    w = gen.plain_config(2000, 128, 12.0, 40, seed=8)
+    G = np.random.default_rng(2).normal(size=(3, w.len, w.n)).astype(np.float32)
+    out = []
+    for tile in (0, 512):
+        c = dev(w.charges)
+        rows = gpu.seed_rows(dev(w.seeds), w.len)
+        launch = gpu.make_launch(2, tile, 256)
+        ck = torch.empty(gpu.ckpt_size(w.n, w.len, 10), dtype=torch.float32, device="cuda")
+        gpu.trace_into(c, rows, w.n, rules.angle(), launch, ck, 10)
+        gc, _ = gpu.adjoint(c, rows, ck, 10, dev(G), rules.angle(), launch)
+        out.append(gc.cpu().numpy().astype(np.float64))
+    assert rel(out[1], out[0]) < 1e-3
+
+
+def _config3_problem(seed=0, sigma=0.05):
+    w = gen.config3(seed)
+    target_q = gen.perturbed(w.charges, sigma, seed=seed)
+    return w, target_q
+
+
+def test_config3_gradient_vs_fp64_adjoint_and_fd(gpu, orc):
+    """256 charges, 2,048 lines, 500 rows, checkpoints every 50 steps."""
+    rule = rules.angle()
+    w, tq = _config3_problem()
+    npts = w.n * w.len
+    tgt = gpu.trace(dev(tq), dev(w.seeds), w.len, rule)
+    c = dev(w.charges).requires_grad_(True)
+    rows = gpu.trace_lines(c, dev(w.seeds), w.len, rule, 50)
+    loss = ((rows - tgt) ** 2).sum() / npts
+    loss.backward()
+    torch.cuda.synchronize()
+    g_gpu = c.grad.cpu().numpy().astype(np.float64)
+    G = (2.0 / npts) * (rows.detach() - tgt).cpu().numpy().astype(np.float64)
+    G64 = np.ascontiguousarray(G.transpose(1, 2, 0))
+    g64, _ = orc.adjoint64(w.charges, w.seeds, w.len, rule, G64)
+    r_adj = rel(g_gpu, g64)
+    # FP64 central differences of the FP64 loss on sampled parameters
+    tgt64 = orc.trace64(tq, w.seeds, w.len, rule)
+    base = w.charges.astype(np.float64)
+
+    def loss64(ch):
+        r = orc.trace64(ch, w.seeds, w.len, rule)
+        return float(((r - tgt64) ** 2).sum() / npts)
+
+    rng = np.random.default_rng(7)
+    big = np.argsort(-np.abs(g64).ravel())[:6]
+    picks = np.unique(np.concatenate([big, rng.choice(g64.size, 4, replace=False)]))
+    h = 1e-4
+    fd = []
+    for flat in picks:
+        i, k = divmod(int(flat), 4)
+        cp, cm = base.copy(), base.copy()
+        cp[i, k] += h
+        cm[i, k] -= h
+        fd.append((loss64(cp) - loss64(cm)) / (2 * h))
+    fd = np.array(fd)
+    gsel = g_gpu.ravel()[picks]
+    r_fd = float(np.linalg.norm(gsel - fd) / np.linalg.norm(fd))
+    r_fd64 = float(np.linalg.norm(g64.ravel()[picks] - fd) / np.linalg.norm(fd))
+    print(f"[config3] loss {loss.item():.4e}; |g| {np.linalg.norm(g64):.3e}; rel(gpu, fp64 adjoint) "
+          f"{r_adj:.2e}; rel(gpu, fp64 FD) {r_fd:.2e}; rel(fp64 adjoint, FD) {r_fd64:.2e}", flush=True)
+    assert r_adj < 1e-2
+    assert r_fd < 2e-2
+
+
+def test_autograd_seed_gradient_and_grad_of_row0(gpu):
+    w = gen.plain_config(64, 64, 4.0, 30, seed=2)
+    s = dev(w.seeds).requires_grad_(True)
+    c = dev(w.charges).requires_grad_(True)
+    rows = gpu.trace_lines(c, s, w.len, rules.plain(10.0), 5)
+    (rows[:, 0, :].sum() * 0 + rows.sum()).backward()
+    assert s.grad.shape == (w.n, 3) and torch.isfinite(s.grad).all()
+    assert c.grad.shape == (w.p, 4) and torch.isfinite(c.grad).all()
+
+
+def test_ref_rule_adjoint_is_refused(gpu):
+    w = gen.plain_config(16, 64, 4.0, 10, seed=2)
+    c = dev(w.charges)
+    rows = gpu.seed_rows(dev(w.seeds), w.len)
+    ck = torch.empty(gpu.ckpt_size(w.n, w.len, 5), dtype=torch.float32, device="cuda")
+    gpu.trace_into(c, rows, w.n, rules.ref(10.0), None, ck, 5)
+    with pytest.raises(gpu.EmError) as e:
+        gpu.adjoint(c, rows, ck, 5, torch.zeros_like(rows), rules.ref(10.0))
+    assert e.value.code == 20
