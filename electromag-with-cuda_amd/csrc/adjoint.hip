@@ -146,7 +146,8 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
     const float4* ch = charges + cfg.charge_off;
     const uint32_t p = cfg.p;
     const uint32_t pp = round_up(p, S * kGroup);
-    if (pp <= tile_cap) {
+    const bool single = pp <= tile_cap;
+    if (single) {
         stage(tile, ch, p, pp);
         __syncthreads();
     }
@@ -166,17 +167,26 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
         st.px = c0.x; st.py = c0.y; st.pz = c0.z; st.h = c0.w;
         st.ux = c1.x; st.uy = c1.y; st.uz = c1.z;
         st.Px = st.Py = st.Pz = 0.f;
+        st.alive = valid && c1.w != 0.f;
+        st.steps = 0;
         const float u0x = c1.x, u0y = c1.y, u0z = c1.z;
-        // recompute the segment, keeping what the backward sweep needs
+        // recompute the segment exactly as the forward ran it, keeping what the
+        // backward sweep needs
         for (uint32_t s = s0 + 1; s <= s1; ++s) {
-            float ex, ey, ez;
-            field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
-            ex *= sp.scale;
-            ey *= sp.scale;
-            ez *= sp.scale;
             const float xx = st.px, xy = st.py, xz = st.pz;
             StepRec r;
-            step_update(sp, s, ex, ey, ez, st, &r);
+            r.L = 1.f; r.c = 0.f; r.h = 0.f; r.branch = 4; r.ux = r.uy = r.uz = 0.f;
+            const bool need = single ? __any(st.alive) : __syncthreads_or(st.alive);
+            if (need) {
+                float ex, ey, ez;
+                field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
+                if (st.alive) {
+                    ex *= sp.scale;
+                    ey *= sp.scale;
+                    ez *= sp.scale;
+                    step_update(sp, s, ex, ey, ez, st, &r);
+                }
+            }
             const uint32_t j = s - s0 - 1;
             *rec.at(j, 0) = make_float4(xx, xy, xz, r.L);
             *rec.at(j, 1) = make_float4(r.ux, r.uy, r.uz, r.h);
@@ -190,46 +200,55 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
             lx += GX[o];
             ly += GY[o];
             lz += GZ[o];
-            const float ux = r1.x, uy = r1.y, uz = r1.z, h = r1.w;
-            lh += lx * ux + ly * uy + lz * uz;
-            lux += h * lx;
-            luy += h * ly;
-            luz += h * lz;
+            const bool moved = r2.y != 4.f;  // x_s = x_{s-1} once the line has ended
+            float wx = 0.f, wy = 0.f, wz = 0.f;
             float lpx = 0.f, lpy = 0.f, lpz = 0.f, hpb = 0.f;
-            if (sp.rule == kRuleAngle && s >= 2 && r2.y == 0.f) {
-                float vx, vy, vz;
-                if (j > 0) {
-                    const float4 q1 = *rec.at(j - 1, 1);
-                    vx = q1.x; vy = q1.y; vz = q1.z;
-                } else {
-                    vx = u0x; vy = u0y; vz = u0z;
+            if (moved) {
+                const float ux = r1.x, uy = r1.y, uz = r1.z, h = r1.w;
+                lh += lx * ux + ly * uy + lz * uz;
+                lux += h * lx;
+                luy += h * ly;
+                luz += h * lz;
+                if (sp.rule == kRuleAngle && s >= 2 && r2.y == 0.f) {
+                    float vx, vy, vz;
+                    if (j > 0) {
+                        const float4 q1 = *rec.at(j - 1, 1);
+                        vx = q1.x; vy = q1.y; vz = q1.z;
+                    } else {
+                        vx = u0x; vy = u0y; vz = u0z;
+                    }
+                    const float c = r2.x;
+                    const float cb = -lh * h / c;
+                    hpb = lh * sp.turn / c;
+                    const float dvx = (ux - vx) / c, dvy = (uy - vy) / c, dvz = (uz - vz) / c;
+                    lux += cb * dvx;
+                    luy += cb * dvy;
+                    luz += cb * dvz;
+                    lpx = -cb * dvx;
+                    lpy = -cb * dvy;
+                    lpz = -cb * dvz;
                 }
-                const float c = r2.x;
-                const float cb = -lh * h / c;
-                hpb = lh * sp.turn / c;
-                const float dvx = (ux - vx) / c, dvy = (uy - vy) / c, dvz = (uz - vz) / c;
-                lux += cb * dvx;
-                luy += cb * dvy;
-                luz += cb * dvz;
-                lpx = -cb * dvx;
-                lpy = -cb * dvy;
-                lpz = -cb * dvz;
+                const float ud = ux * lux + uy * luy + uz * luz;
+                const float L = r0.w;
+                wx = sp.scale * ((lux - ux * ud) / L);
+                wy = sp.scale * ((luy - uy * ud) / L);
+                wz = sp.scale * ((luz - uz * ud) / L);
             }
-            const float ud = ux * lux + uy * luy + uz * luz;
-            const float L = r0.w;
-            const float wx = sp.scale * ((lux - ux * ud) / L);
-            const float wy = sp.scale * ((luy - uy * ud) / L);
-            const float wz = sp.scale * ((luz - uz * ud) / L);
             if (writer) {
                 WX[o] = wx;
                 WY[o] = wy;
                 WZ[o] = wz;
             }
-            float jx, jy, jz;
-            jvp_at<S>(tile, tile_cap, ch, p, k, r0.x, r0.y, r0.z, wx, wy, wz, jx, jy, jz);
-            lx += jx;
-            ly += jy;
-            lz += jz;
+            const bool need = single ? __any(moved) : __syncthreads_or(moved);
+            if (need) {
+                float jx, jy, jz;
+                jvp_at<S>(tile, tile_cap, ch, p, k, r0.x, r0.y, r0.z, wx, wy, wz, jx, jy, jz);
+                if (moved) {
+                    lx += jx;
+                    ly += jy;
+                    lz += jz;
+                }
+            }
             lux = lpx;
             luy = lpy;
             luz = lpz;
@@ -239,9 +258,10 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
     if (writer && gseed) {
         const uint64_t o = cfg.line_off + li;
         const uint64_t plane = (uint64_t)cfg.grad_line_off;  // seeds gradient plane stride
-        gseed[line] = lx + GX[o];
-        gseed[plane + line] = ly + GY[o];
-        gseed[2 * plane + line] = lz + GZ[o];
+        const uint64_t g = cfg.line_index + line;
+        gseed[g] = lx + GX[o];
+        gseed[plane + g] = ly + GY[o];
+        gseed[2 * plane + g] = lz + GZ[o];
     }
 }
 

@@ -38,6 +38,8 @@ struct em_batch {
     hipGraphExec_t exec = nullptr;
     const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t graph_every = 0;
+    hipStream_t cap_stream = nullptr;  // graphs are recorded here (the caller's stream may be
+                                       // the null stream, which cannot be captured)
     em::AdjPlan* adj = nullptr;  // adjoint work lists, built on the first adjoint call
     uint32_t adj_every = 0;
 };
@@ -53,6 +55,8 @@ static int free_plan(em_batch* b)
     if (b->d_cfgs) EM_HIP(hipFree(b->d_cfgs));
     if (b->d_items) EM_HIP(hipFree(b->d_items));
     if (b->d_counter) EM_HIP(hipFree(b->d_counter));
+    if (b->cap_stream) EM_HIP(hipStreamDestroy(b->cap_stream));
+    b->cap_stream = nullptr;
     b->exec = nullptr;
     b->graph = nullptr;
     b->d_cfgs = nullptr;
@@ -77,7 +81,7 @@ static int set_ckpt_offsets(em_batch* b, uint32_t every, hipStream_t stream)
 }
 
 static int run_plan(em_batch* b, const float* d_charges, float* d_x, float* d_y, float* d_z,
-                    float* d_ckpt, uint32_t every, hipStream_t stream)
+                    float* d_ckpt, uint32_t every, uint32_t* d_steps, hipStream_t stream)
 {
     int dev = 0;
     EM_HIP(hipGetDevice(&dev));
@@ -87,19 +91,21 @@ static int run_plan(em_batch* b, const float* d_charges, float* d_x, float* d_y,
     const uint32_t ev = every ? every : 1;
     if (!b->use_graph)
         return launch_trace_persistent(b->sh, b->grid, b->d_cfgs, b->d_items, (uint32_t)b->items.size(),
-                                       b->d_counter, ch, d_x, d_y, d_z, b->sp, d_ckpt, ev, stream);
-    const void* key[6] = {d_charges, d_x, d_y, d_z, d_ckpt, stream};
+                                       b->d_counter, ch, d_x, d_y, d_z, b->sp, d_ckpt, ev, d_steps,
+                                       stream);
+    const void* key[6] = {d_charges, d_x, d_y, d_z, d_ckpt, d_steps};
     if (!b->exec || std::memcmp(key, b->graph_key, sizeof(key)) != 0 || b->graph_every != ev) {
         if (b->exec) EM_HIP(hipGraphExecDestroy(b->exec));
         if (b->graph) EM_HIP(hipGraphDestroy(b->graph));
         b->exec = nullptr;
         b->graph = nullptr;
-        EM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        if (!b->cap_stream) EM_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        EM_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
         int rc = launch_trace_persistent(b->sh, b->grid, b->d_cfgs, b->d_items,
                                          (uint32_t)b->items.size(), b->d_counter, ch, d_x, d_y, d_z,
-                                         b->sp, d_ckpt, ev, stream);
+                                         b->sp, d_ckpt, ev, d_steps, b->cap_stream);
         hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(stream, &g);
+        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -175,6 +181,7 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
     }
 
     const uint32_t lpb = (uint32_t)(sh.block / sh.S);
+    uint64_t line_index = 0;
     b->cfgs.resize(count);
     for (size_t i = 0; i < count; ++i) {
         const em_batch_cfg& c = cfgs[i];
@@ -186,6 +193,8 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
         d.p = c.p;
         d.n = c.n;
         d.len = c.len;
+        d.line_index = line_index;
+        line_index += c.n;
         b->interactions += (double)c.p * (double)c.n * (double)(c.len - 1);
         for (uint32_t l0 = 0; l0 < c.n; l0 += lpb) b->items.push_back(Item{(uint32_t)i, l0});
     }
@@ -223,11 +232,12 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
 }
 
 int em_batch_run(em_batch* plan, const float* d_charges, float* d_x, float* d_y, float* d_z,
-                 void* stream)
+                 uint32_t* d_steps, void* stream)
 {
     clear_error();
     if (!plan || !d_charges || !d_x || !d_y || !d_z) EM_FAIL(EM_ERR_INVALID, "NULL argument");
-    return run_plan(plan, d_charges, d_x, d_y, d_z, nullptr, 0, static_cast<hipStream_t>(stream));
+    return run_plan(plan, d_charges, d_x, d_y, d_z, nullptr, 0, d_steps,
+                    static_cast<hipStream_t>(stream));
 }
 
 int em_batch_info(const em_batch* plan, double* interactions, int32_t* items, int32_t* grid,
@@ -297,13 +307,13 @@ int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x, c
 }
 
 int em_batch_run_ckpt(em_batch* plan, const float* d_charges, float* d_x, float* d_y, float* d_z,
-                      float* d_ckpt, uint32_t ckpt_every, void* stream)
+                      float* d_ckpt, uint32_t ckpt_every, uint32_t* d_steps, void* stream)
 {
     clear_error();
     if (!plan || !d_charges || !d_x || !d_y || !d_z || !d_ckpt)
         EM_FAIL(EM_ERR_INVALID, "NULL argument");
     if (!ckpt_every) EM_FAIL(EM_ERR_INVALID, "ckpt_every == 0");
-    return run_plan(plan, d_charges, d_x, d_y, d_z, d_ckpt, ckpt_every,
+    return run_plan(plan, d_charges, d_x, d_y, d_z, d_ckpt, ckpt_every, d_steps,
                     static_cast<hipStream_t>(stream));
 }
 

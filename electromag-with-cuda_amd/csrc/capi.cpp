@@ -127,6 +127,8 @@ int validate_step(const em_step* st)
     } else {
         EM_FAIL(EM_ERR_INVALID, "unknown step rule %d", st->rule);
     }
+    if (st->stop && st->rule != kRuleAngle)
+        EM_FAIL(EM_ERR_INVALID, "stop is defined for the angle rule only");
     return EM_OK;
 }
 
@@ -140,6 +142,7 @@ StepP to_stepp(const em_step& st)
     sp.hmax = st.hmax;
     sp.turn = st.turn;
     sp.scale = (st.rule == kRuleAngle) ? 1.0f : (float)8.987551787E9;  // Electrostatics.h:27
+    sp.stop = st.stop ? 1 : 0;
     return sp;
 }
 
@@ -161,6 +164,8 @@ struct HostPathBuffers {
     size_t charges_bytes = 0;
     float* lines = nullptr;  // 3 * len * n floats
     size_t lines_bytes = 0;
+    void* steps = nullptr;   // n step counts
+    size_t steps_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -193,7 +198,8 @@ static void fill_perf(em_perf* perf, double p, double n, double len, double kern
 }
 
 static int host_trace(const float* charges, size_t p, float* x, float* y, float* z, size_t n,
-                      size_t len, const em_step* st, const em_launch* launch, em_perf* perf)
+                      size_t len, const em_step* st, const em_launch* launch, em_perf* perf,
+                      uint32_t* steps)
 {
     const double t0 = now_s();
     if (perf) {
@@ -219,6 +225,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     const size_t row = n * sizeof(float);
     EM_TRY(ensure(&b.charges, &b.charges_bytes, cb));
     EM_TRY(ensure(reinterpret_cast<void**>(&b.lines), &b.lines_bytes, 3 * len * row));
+    if (steps) EM_TRY(ensure(&b.steps, &b.steps_bytes, n * sizeof(uint32_t)));
     float* dx = b.lines;
     float* dy = dx + len * n;
     float* dz = dy + len * n;
@@ -238,13 +245,15 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     cfg.n = (uint32_t)n;
     cfg.len = (uint32_t)len;
     const StepP sp = to_stepp(*st);
+    uint32_t* dsteps = steps ? static_cast<uint32_t*>(b.steps) : nullptr;
     EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
-                               nullptr, 1, b.stream));
+                               nullptr, 1, dsteps, b.stream));
     EM_HIP(hipEventRecord(b.ev[2], b.stream));
     const size_t out_bytes = (len - 1) * row;
     EM_HIP(hipMemcpyAsync(x + n, dx + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
     EM_HIP(hipMemcpyAsync(y + n, dy + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
     EM_HIP(hipMemcpyAsync(z + n, dz + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
+    if (steps) EM_HIP(hipMemcpyAsync(steps, dsteps, n * sizeof(uint32_t), hipMemcpyDeviceToHost, b.stream));
     EM_HIP(hipEventRecord(b.ev[3], b.stream));
     EM_HIP(hipEventSynchronize(b.ev[3]));
 
@@ -313,6 +322,7 @@ int em_shutdown(void)
         EM_HIP(hipSetDevice(i));
         if (b.charges) EM_HIP(hipFree(b.charges));
         if (b.lines) EM_HIP(hipFree(b.lines));
+        if (b.steps) EM_HIP(hipFree(b.steps));
         for (auto& e : b.ev)
             if (e) EM_HIP(hipEventDestroy(e));
         if (b.stream) EM_HIP(hipStreamDestroy(b.stream));
@@ -329,14 +339,15 @@ int em_calc_lines_f32(const float* charges, size_t p, float* x, float* y, float*
     em_step st{};
     st.rule = use_curvature ? EM_RULE_REF : EM_RULE_PLAIN;
     st.resolution = resolution;
-    return host_trace(charges, p, x, y, z, n, len, &st, nullptr, perf);
+    return host_trace(charges, p, x, y, z, n, len, &st, nullptr, perf, nullptr);
 }
 
 int em_trace_host_f32(const float* charges, size_t p, float* x, float* y, float* z, size_t n,
-                      size_t len, const em_step* step, const em_launch* launch, em_perf* perf)
+                      size_t len, const em_step* step, const em_launch* launch, em_perf* perf,
+                      uint32_t* steps)
 {
     clear_error();
-    return host_trace(charges, p, x, y, z, n, len, step, launch, perf);
+    return host_trace(charges, p, x, y, z, n, len, step, launch, perf, steps);
 }
 
 size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every)
@@ -347,7 +358,8 @@ size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every)
 
 int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, float* d_z,
                      size_t n, size_t len, size_t pitch, const em_step* step,
-                     const em_launch* launch, float* d_ckpt, uint32_t ckpt_every, void* stream)
+                     const em_launch* launch, float* d_ckpt, uint32_t ckpt_every,
+                     uint32_t* d_steps, void* stream)
 {
     clear_error();
     EM_TRY(check_sizes(p, n, len, step));
@@ -364,7 +376,7 @@ int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, f
     cfg.n = (uint32_t)n;
     cfg.len = (uint32_t)len;
     return launch_trace_direct(sh, cfg, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
-                               to_stepp(*step), d_ckpt, ckpt_every ? ckpt_every : 1,
+                               to_stepp(*step), d_ckpt, ckpt_every ? ckpt_every : 1, d_steps,
                                static_cast<hipStream_t>(stream));
 }
 

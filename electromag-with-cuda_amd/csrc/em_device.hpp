@@ -39,7 +39,7 @@ struct Cfg {
     uint64_t line_off;    // element offset of (row 0, line 0) in x/y/z
     uint64_t pitch;       // row pitch (elements)
     uint64_t ckpt_off;    // first float of this configuration's checkpoints
-    uint64_t src_off;     // first record of this configuration's adjoint sources
+    uint64_t line_index;  // global index of this configuration's first line (step counts)
     uint32_t p, n, len, grad_line_off;
 };
 
@@ -52,6 +52,7 @@ struct StepP {
     int rule;
     float res, h0, hmin, hmax, turn;
     float scale;  // factor applied to the Coulomb sum (k_e for plain/ref)
+    int stop;     // angle rule: end a line when its direction reverses
 };
 
 __host__ __device__ constexpr uint32_t round_up(uint32_t a, uint32_t b) { return (a + b - 1) / b * b; }
@@ -223,6 +224,8 @@ struct LineState {
     float ux, uy, uz;  // previous unit direction (angle rule)
     float h;           // previous step length (angle rule)
     float Px, Py, Pz;  // seed point (ref rule: the SSE path's prevAccum)
+    bool alive;        // false once the line has ended (stop rule)
+    uint32_t steps;    // steps taken (rows that moved the line)
 };
 
 // Record of one step, kept by the adjoint's recompute.
@@ -230,7 +233,8 @@ struct StepRec {
     float L;       // |E|
     float c;       // chord |u - u_prev|
     float h;       // step taken
-    int branch;    // 0: turn ratio, 1: clamp hmax, 2: clamp hmin, 3: constant
+    int branch;    // 0: turn ratio, 1: clamp hmax, 2: clamp hmin, 3: constant,
+                   // 4: the line ended here or earlier (no move)
     float ux, uy, uz;
 };
 
@@ -266,6 +270,12 @@ __device__ __forceinline__ void step_update(const StepP& sp, uint32_t s, float e
     } else {
         const float len = sqrtf(ex * ex + ey * ey + ez * ez);
         const float ux = ex / len, uy = ey / len, uz = ez / len;
+        if (sp.stop && s > 1 && ux * st.ux + uy * st.uy + uz * st.uz < 0.f) {
+            // the direction reversed: the line went through a sink; it ends at x_{s-1}
+            st.alive = false;
+            if (rec) rec->branch = 4;
+            return;
+        }
         float h = sp.h0, c = 0.f;
         int branch = 3;
         if (s > 1) {

@@ -67,11 +67,11 @@ int validate_step(const em_step* st);
 // ---- kernels (defined in trace.hip / adjoint.hip) --------------------------
 int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                         float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
-                        hipStream_t stream);
+                        uint32_t* steps, hipStream_t stream);
 int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const Item* items,
                             uint32_t n_items, uint32_t* counter, const float4* charges, float* x,
                             float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
-                            hipStream_t stream);
+                            uint32_t* steps, hipStream_t stream);
 int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid);
 int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,
                  const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,

@@ -26,7 +26,7 @@ __device__ __forceinline__ void store_ckpt(float* ckpt, uint64_t off, uint32_t m
 {
     float4* c = reinterpret_cast<float4*>(ckpt + off) + 2ull * ((uint64_t)m * n + line);
     c[0] = make_float4(st.px, st.py, st.pz, st.h);
-    c[1] = make_float4(st.ux, st.uy, st.uz, 0.f);
+    c[1] = make_float4(st.ux, st.uy, st.uz, st.alive ? 1.f : 0.f);
 }
 
 // Trace lines [line0, line0 + blockDim/S) of one configuration.
@@ -35,7 +35,8 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
                                             const float4* __restrict__ charges, float* __restrict__ X,
                                             float* __restrict__ Y, float* __restrict__ Z,
                                             const StepP& sp, float4* tile, uint32_t tile_cap,
-                                            float* __restrict__ ckpt, uint32_t ckpt_every)
+                                            float* __restrict__ ckpt, uint32_t ckpt_every,
+                                            uint32_t* __restrict__ steps_out)
 {
     const int tid = threadIdx.x;
     const int k = tid % S;
@@ -45,7 +46,8 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
     const float4* ch = charges + cfg.charge_off;
     const uint32_t p = cfg.p;
     const uint32_t pp = round_up(p, S * kGroup);
-    if (pp <= tile_cap) {
+    const bool single = pp <= tile_cap;
+    if (single) {
         stage(tile, ch, p, pp);
         __syncthreads();
     }
@@ -57,15 +59,26 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
     st.Px = st.px; st.Py = st.py; st.Pz = st.pz;
     st.ux = st.uy = st.uz = 0.f;
     st.h = sp.h0;
+    st.alive = valid;
+    st.steps = 0;
     const bool writer = valid && k == 0;
     if (ckpt && writer) store_ckpt(ckpt, cfg.ckpt_off, 0, cfg.n, line, st);
     for (uint32_t s = 1; s < cfg.len; ++s) {
-        float ex, ey, ez;
-        field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
-        ex *= sp.scale;
-        ey *= sp.scale;
-        ez *= sp.scale;
-        step_update(sp, s, ex, ey, ez, st);
+        // Lines that ended keep writing their end point; the charge loop runs
+        // while any line of the wave (of the block, when charges are re-staged
+        // with barriers) is still moving.
+        const bool need = single ? __any(st.alive) : __syncthreads_or(st.alive);
+        if (need) {
+            float ex, ey, ez;
+            field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
+            if (st.alive) {
+                ex *= sp.scale;
+                ey *= sp.scale;
+                ez *= sp.scale;
+                step_update(sp, s, ex, ey, ez, st);
+                if (st.alive) st.steps = s;
+            }
+        }
         if (writer) {
             const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + line;
             __builtin_nontemporal_store(st.px, X + o);
@@ -74,6 +87,7 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
             if (ckpt && s % ckpt_every == 0) store_ckpt(ckpt, cfg.ckpt_off, s / ckpt_every, cfg.n, line, st);
         }
     }
+    if (writer && steps_out) steps_out[cfg.line_index + line] = st.steps;
 }
 
 template <int S>
@@ -81,11 +95,11 @@ __global__ void __launch_bounds__(512) k_trace_direct(Cfg cfg, const float4* __r
                                                       float* __restrict__ X, float* __restrict__ Y,
                                                       float* __restrict__ Z, StepP sp,
                                                       uint32_t tile_cap, float* __restrict__ ckpt,
-                                                      uint32_t ckpt_every)
+                                                      uint32_t ckpt_every, uint32_t* __restrict__ steps)
 {
     extern __shared__ float4 tile[];
     const uint32_t line0 = blockIdx.x * (blockDim.x / S);
-    trace_block<S>(cfg, line0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every);
+    trace_block<S>(cfg, line0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
 }
 
 template <int S>
@@ -96,7 +110,7 @@ __global__ void __launch_bounds__(512) k_trace_persistent(const Cfg* __restrict_
                                                           float* __restrict__ X, float* __restrict__ Y,
                                                           float* __restrict__ Z, StepP sp,
                                                           uint32_t tile_cap, float* __restrict__ ckpt,
-                                                          uint32_t ckpt_every)
+                                                          uint32_t ckpt_every, uint32_t* __restrict__ steps)
 {
     extern __shared__ float4 tile[];
     __shared__ uint32_t s_item;
@@ -108,7 +122,7 @@ __global__ void __launch_bounds__(512) k_trace_persistent(const Cfg* __restrict_
         if (it >= n_items) break;  // uniform across the block: every wave exits
         const Item item = items[it];
         const Cfg cfg = cfgs[item.cfg];
-        trace_block<S>(cfg, item.line0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every);
+        trace_block<S>(cfg, item.line0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
         __syncthreads();  // the next item re-stages the tile
     }
 }
@@ -169,14 +183,15 @@ int dispatch_S(int S, A&&... a)
 template <int S>
 struct DirectL {
     static int run(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
-                   float* z, const StepP& sp, float* ckpt, uint32_t every, hipStream_t stream)
+                   float* z, const StepP& sp, float* ckpt, uint32_t every, uint32_t* steps,
+                   hipStream_t stream)
     {
         const uint32_t lpb = sh.block / S;
         const uint32_t grid = (cfg.n + lpb - 1) / lpb;
         const size_t lds = (size_t)sh.tile * sizeof(float4);
         EM_TRY(allow_lds(k_trace_direct<S>, lds));
         hipLaunchKernelGGL(k_trace_direct<S>, dim3(grid), dim3(sh.block), lds, stream, cfg, charges,
-                           x, y, z, sp, sh.tile, ckpt, every);
+                           x, y, z, sp, sh.tile, ckpt, every, steps);
         EM_HIP(hipGetLastError());
         return EM_OK;
     }
@@ -186,14 +201,16 @@ template <int S>
 struct PersistentL {
     static int run(const Shape& sh, int grid, const Cfg* cfgs, const Item* items, uint32_t n_items,
                    uint32_t* counter, const float4* charges, float* x, float* y, float* z,
-                   const StepP& sp, float* ckpt, uint32_t every, hipStream_t stream)
+                   const StepP& sp, float* ckpt, uint32_t every, uint32_t* steps,
+                   hipStream_t stream)
     {
         const size_t lds = (size_t)sh.tile * sizeof(float4);
         EM_TRY(allow_lds(k_trace_persistent<S>, lds));
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
         EM_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_trace_persistent<S>, dim3(grid), dim3(sh.block), lds, stream, cfgs,
-                           items, n_items, counter, charges, x, y, z, sp, sh.tile, ckpt, every);
+                           items, n_items, counter, charges, x, y, z, sp, sh.tile, ckpt, every,
+                           steps);
         EM_HIP(hipGetLastError());
         return EM_OK;
     }
@@ -236,18 +253,18 @@ struct FieldL {
 
 int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                         float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
-                        hipStream_t stream)
+                        uint32_t* steps, hipStream_t stream)
 {
-    return dispatch_S<DirectL>(sh.S, sh, cfg, charges, x, y, z, sp, ckpt, ckpt_every, stream);
+    return dispatch_S<DirectL>(sh.S, sh, cfg, charges, x, y, z, sp, ckpt, ckpt_every, steps, stream);
 }
 
 int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const Item* items,
                             uint32_t n_items, uint32_t* counter, const float4* charges, float* x,
                             float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
-                            hipStream_t stream)
+                            uint32_t* steps, hipStream_t stream)
 {
     return dispatch_S<PersistentL>(sh.S, sh, grid, cfgs, items, n_items, counter, charges, x, y, z,
-                                   sp, ckpt, ckpt_every, stream);
+                                   sp, ckpt, ckpt_every, steps, stream);
 }
 
 int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid)

@@ -38,7 +38,7 @@ class EmError(RuntimeError):
 class em_step(ctypes.Structure):
     _fields_ = [("rule", ctypes.c_int32), ("resolution", ctypes.c_float),
                 ("h0", ctypes.c_float), ("hmin", ctypes.c_float),
-                ("hmax", ctypes.c_float), ("turn", ctypes.c_float)]
+                ("hmax", ctypes.c_float), ("turn", ctypes.c_float), ("stop", ctypes.c_int32)]
 
 
 class em_launch(ctypes.Structure):
@@ -80,21 +80,21 @@ SIGNATURES = {
     "em_version": (ctypes.c_char_p, []),
     "em_calc_lines_f32": (I, [P, SZ, P, P, P, SZ, SZ, F, I, ctypes.POINTER(em_perf)]),
     "em_trace_host_f32": (I, [P, SZ, P, P, P, SZ, SZ, ctypes.POINTER(em_step),
-                              ctypes.POINTER(em_launch), ctypes.POINTER(em_perf)]),
+                              ctypes.POINTER(em_launch), ctypes.POINTER(em_perf), P]),
     "em_trace_dev_f32": (I, [P, SZ, P, P, P, SZ, SZ, SZ, ctypes.POINTER(em_step),
-                             ctypes.POINTER(em_launch), P, U32, P]),
+                             ctypes.POINTER(em_launch), P, U32, P, P]),
     "em_ckpt_floats": (SZ, [SZ, SZ, U32]),
     "em_field_dev_f32": (I, [P, SZ, P, P, P, SZ, F, P, P, P, ctypes.POINTER(em_launch), P]),
     "em_batch_create": (I, [ctypes.POINTER(em_batch_cfg), SZ, ctypes.POINTER(em_step),
                             ctypes.POINTER(em_launch), I, ctypes.POINTER(P)]),
-    "em_batch_run": (I, [P, P, P, P, P, P]),
+    "em_batch_run": (I, [P, P, P, P, P, P, P]),
     "em_batch_info": (I, [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                           ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(em_launch)]),
     "em_batch_destroy": (I, [P]),
     "em_adjoint_dev_f32": (I, [P, SZ, P, P, P, SZ, SZ, SZ, ctypes.POINTER(em_step),
                                ctypes.POINTER(em_launch), P, U32, P, P, P, P, P, P]),
     "em_batch_ckpt_floats": (SZ, [P, U32]),
-    "em_batch_run_ckpt": (I, [P, P, P, P, P, P, U32, P]),
+    "em_batch_run_ckpt": (I, [P, P, P, P, P, P, U32, P, P]),
     "em_batch_adjoint": (I, [P, P, P, P, P, P, U32, P, P, P, P, P]),
 }
 

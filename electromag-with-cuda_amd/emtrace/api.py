@@ -23,7 +23,7 @@ from .rules import ANGLE, PLAIN, REF, StepRule
 
 def make_step(rule: StepRule) -> em_step:
     return em_step(int(rule.rule), float(rule.resolution), float(rule.h0), float(rule.hmin),
-                   float(rule.hmax), float(rule.turn))
+                   float(rule.hmax), float(rule.turn), int(bool(rule.stop)))
 
 
 def make_launch(lanes_per_line: int = 0, tile: int = 0, block: int = 0) -> em_launch:
@@ -65,18 +65,19 @@ def calc_lines(charges: np.ndarray, x: np.ndarray, y: np.ndarray, z: np.ndarray,
 
 
 def trace_host(charges: np.ndarray, seeds: np.ndarray, length: int, rule: StepRule, launch=None):
-    """Trace on the GPU from host arrays.  Returns (rows[3, len, n], perf dict)."""
+    """Trace on the GPU from host arrays.  Returns (rows[3, len, n], perf dict, steps[n])."""
     charges = np.ascontiguousarray(charges, dtype=np.float32)
     seeds = np.asarray(seeds, dtype=np.float32)
     n = seeds.shape[0]
     rows = np.zeros((3, length, n), dtype=np.float32)
     rows[:, 0, :] = seeds.T
+    steps = np.zeros(n, dtype=np.uint32)
     perf = em_perf()
     st = make_step(rule)
     call("em_trace_host_f32", _np_ptr(charges), charges.shape[0], _np_ptr(rows[0]),
          _np_ptr(rows[1]), _np_ptr(rows[2]), n, length, ctypes.byref(st), _launch_arg(launch),
-         ctypes.byref(perf))
-    return rows, perf.as_dict()
+         ctypes.byref(perf), _np_ptr(steps))
+    return rows, perf.as_dict(), steps
 
 
 # ---------------------------------------------------------------------------
@@ -115,8 +116,10 @@ def ckpt_size(n: int, length: int, every: int) -> int:
     return int(lib().em_ckpt_floats(n, length, every))
 
 
-def trace_into(charges, rows, n: int, rule: StepRule, launch=None, ckpt=None, ckpt_every: int = 0):
-    """Trace in place on device rows (3, len, pitch); row 0 must hold the seeds."""
+def trace_into(charges, rows, n: int, rule: StepRule, launch=None, ckpt=None, ckpt_every: int = 0,
+               steps=None):
+    """Trace in place on device rows (3, len, pitch); row 0 must hold the seeds.
+    ``steps`` (optional int32 device tensor of n) receives each line's step count."""
     torch = _torch()
     _dev_check(charges, "charges", torch.float32)
     _dev_check(rows, "rows", torch.float32)
@@ -132,16 +135,28 @@ def trace_into(charges, rows, n: int, rule: StepRule, launch=None, ckpt=None, ck
         if ckpt.numel() < ckpt_size(n, length, ckpt_every):
             raise ValueError("checkpoint buffer too small")
         cp = ckpt.data_ptr()
+    sp = None
+    if steps is not None:
+        _dev_check(steps, "steps", torch.int32)
+        if steps.numel() < n:
+            raise ValueError("steps buffer too small")
+        sp = steps.data_ptr()
     call("em_trace_dev_f32", charges.data_ptr(), charges.shape[0], rows[0].data_ptr(),
          rows[1].data_ptr(), rows[2].data_ptr(), n, length, pitch, ctypes.byref(st),
-         _launch_arg(launch), cp, ckpt_every, _stream_handle())
+         _launch_arg(launch), cp, ckpt_every, sp, _stream_handle())
     return rows
 
 
-def trace(charges, seeds, length: int, rule: StepRule, launch=None):
-    """Trace from device seeds (n, 3).  Returns rows (3, len, n) on the device."""
+def trace(charges, seeds, length: int, rule: StepRule, launch=None, with_steps: bool = False):
+    """Trace from device seeds (n, 3).  Returns rows (3, len, n) on the device
+    [and the int32 step count of every line]."""
     rows = seed_rows(seeds, length)
-    return trace_into(charges, rows, seeds.shape[0], rule, launch)
+    if not with_steps:
+        return trace_into(charges, rows, seeds.shape[0], rule, launch)
+    torch = _torch()
+    steps = torch.empty(seeds.shape[0], dtype=torch.int32, device=seeds.device)
+    trace_into(charges, rows, seeds.shape[0], rule, launch, steps=steps)
+    return rows, steps
 
 
 def field(charges, points, scale: float = 1.0, launch=None):
@@ -218,21 +233,30 @@ class BatchPlan:
         return {"interactions": inter.value, "items": items.value, "grid": grid.value,
                 "lanes_per_line": used.lanes_per_line, "tile": used.tile, "block": used.block}
 
-    def run(self, charges, lines):
+    def run(self, charges, lines, steps=None):
+        """Trace every configuration; ``steps`` (optional int32 device tensor of
+        sum(n)) receives the step counts, configurations back to back."""
         torch = _torch()
         _dev_check(charges, "charges", torch.float32)
         _dev_check(lines, "lines", torch.float32)
         if charges.shape[0] < self.layout.total_charges or lines.shape[1] < self.layout.total_line_elems:
             raise ValueError("packed buffers smaller than the layout")
+        sp = None
+        if steps is not None:
+            _dev_check(steps, "steps", torch.int32)
+            if steps.numel() < int(self.layout.n.sum()):
+                raise ValueError("steps buffer too small")
+            sp = steps.data_ptr()
         call("em_batch_run", self._h, charges.data_ptr(), lines[0].data_ptr(), lines[1].data_ptr(),
-             lines[2].data_ptr(), _stream_handle())
+             lines[2].data_ptr(), sp, _stream_handle())
 
     def ckpt_size(self, every: int) -> int:
         return int(lib().em_batch_ckpt_floats(self._h, every))
 
-    def run_ckpt(self, charges, lines, ckpt, every: int):
+    def run_ckpt(self, charges, lines, ckpt, every: int, steps=None):
         call("em_batch_run_ckpt", self._h, charges.data_ptr(), lines[0].data_ptr(),
-             lines[1].data_ptr(), lines[2].data_ptr(), ckpt.data_ptr(), every, _stream_handle())
+             lines[1].data_ptr(), lines[2].data_ptr(), ckpt.data_ptr(), every,
+             None if steps is None else steps.data_ptr(), _stream_handle())
 
     def adjoint(self, charges, lines, ckpt, every: int, grad_lines, grad_charges):
         call("em_batch_adjoint", self._h, charges.data_ptr(), lines[0].data_ptr(),

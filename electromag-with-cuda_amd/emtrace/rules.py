@@ -7,9 +7,13 @@
   point, x += E/(|E|*(k+1)*resolution), E including k_e = 8.987551787e9.
 * ``ANGLE`` -- curvature-adaptive rule with no reference counterpart: the chord
   c = |u_s - u_{s-1}| between successive unit field directions bounds the step,
-  h_s = clamp(turn*h_{s-1}/c, hmin, hmax), h_1 = h0.
+  h_s = clamp(turn*h_{s-1}/c, hmin, hmax), h_1 = h0.  With ``stop`` (the
+  default for this rule) a line ends at the first step whose direction
+  reverses (u_s . u_{s-1} < 0: it went through a sink); its remaining rows
+  repeat the end point and its step count is the last step that moved it.
 
-All rules take exactly len-1 steps per line; row 0 holds the seeds.
+Rows always run 0..len-1 (row 0 = seeds); without stop every line takes
+exactly len-1 steps, as in the reference (CPU_Implement.cpp:49,75).
 """
 from __future__ import annotations
 
@@ -28,6 +32,7 @@ class StepRule:
     hmin: float = 0.005
     hmax: float = 0.5
     turn: float = 0.05         # ANGLE: target chord between successive directions
+    stop: bool = False         # ANGLE: end lines whose direction reverses
 
     @property
     def field_scale(self) -> float:
@@ -35,8 +40,8 @@ class StepRule:
         return K_E if self.rule in (PLAIN, REF) else 1.0
 
 
-def angle(h0=0.1, hmin=0.005, hmax=0.5, turn=0.05) -> StepRule:
-    return StepRule(ANGLE, 10.0, h0, hmin, hmax, turn)
+def angle(h0=0.1, hmin=0.005, hmax=0.5, turn=0.05, stop=True) -> StepRule:
+    return StepRule(ANGLE, 10.0, h0, hmin, hmax, turn, bool(stop))
 
 
 def plain(resolution=10.0) -> StepRule:

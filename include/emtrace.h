@@ -50,6 +50,9 @@ typedef struct em_step {
     float h0;         /* angle: first step                              */
     float hmin, hmax; /* angle: clamp                                   */
     float turn;       /* angle: target chord between successive directions */
+    int32_t stop;     /* angle only: 1 = a line ends at the first step whose direction
+                         reverses (u_s . u_{s-1} < 0: it passed through a sink); its
+                         remaining rows repeat the end point.  0 = always len-1 steps. */
 } em_step;
 
 /* Launch shape.  Any field left 0 is chosen by the library (autotuner cache,
@@ -95,21 +98,24 @@ int em_calc_lines_f32(const float* charges, size_t p, float* x, float* y,
                       float* z, size_t n, size_t len, float resolution,
                       int use_curvature, em_perf* perf);
 
-/* Host arrays, any step rule and launch shape (launch may be NULL). */
+/* Host arrays, any step rule and launch shape (launch may be NULL).
+ * steps (may be NULL): n counts of the steps each line took (len-1 unless it
+ * stopped). */
 int em_trace_host_f32(const float* charges, size_t p, float* x, float* y,
                       float* z, size_t n, size_t len, const em_step* step,
-                      const em_launch* launch, em_perf* perf);
+                      const em_launch* launch, em_perf* perf, uint32_t* steps);
 
 /* ---- device-pointer forms (asynchronous on `stream`, a hipStream_t) -------
  * Replaces the OpenCL functor's device path (GPGPU_Segment/src/
  * CL_Electrostatics.cpp:396-513 MainFunctor; Electrostatics.cl.c:282-367).
  * d_ckpt (may be NULL): when non-NULL, the forward also stores the state
  * needed by em_adjoint_dev_f32 every ckpt_every steps; size it with
- * em_ckpt_floats(). */
+ * em_ckpt_floats().  d_steps (may be NULL): n step counts. */
 int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y,
                      float* d_z, size_t n, size_t len, size_t pitch,
                      const em_step* step, const em_launch* launch,
-                     float* d_ckpt, uint32_t ckpt_every, void* stream);
+                     float* d_ckpt, uint32_t ckpt_every, uint32_t* d_steps,
+                     void* stream);
 
 size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every);
 
@@ -141,7 +147,7 @@ typedef struct em_batch em_batch; /* opaque */
 int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
                     const em_launch* launch, int use_graph, em_batch** out);
 int em_batch_run(em_batch* plan, const float* d_charges, float* d_x, float* d_y,
-                 float* d_z, void* stream);
+                 float* d_z, uint32_t* d_steps, void* stream);
 int em_batch_info(const em_batch* plan, double* interactions, int32_t* items,
                   int32_t* grid, em_launch* used);
 int em_batch_destroy(em_batch* plan);
@@ -167,7 +173,7 @@ int em_adjoint_dev_f32(const float* d_charges, size_t p, const float* d_x,
 size_t em_batch_ckpt_floats(const em_batch* plan, uint32_t ckpt_every);
 int em_batch_run_ckpt(em_batch* plan, const float* d_charges, float* d_x,
                       float* d_y, float* d_z, float* d_ckpt,
-                      uint32_t ckpt_every, void* stream);
+                      uint32_t ckpt_every, uint32_t* d_steps, void* stream);
 int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x,
                      const float* d_y, const float* d_z, const float* d_ckpt,
                      uint32_t ckpt_every, const float* d_gx, const float* d_gy,

@@ -42,7 +42,7 @@ K_E = 8.987551787e9
 class _StepParams(ctypes.Structure):
     _fields_ = [("rule", ctypes.c_int), ("resolution", ctypes.c_double),
                 ("h0", ctypes.c_double), ("hmin", ctypes.c_double),
-                ("hmax", ctypes.c_double), ("turn", ctypes.c_double)]
+                ("hmax", ctypes.c_double), ("turn", ctypes.c_double), ("stop", ctypes.c_int)]
 
 
 def build(ref: bool = True) -> None:
@@ -64,15 +64,18 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(_LIB_PATH)
         P = ctypes.c_void_p
         S = ctypes.c_size_t
-        L.orc_calc_lines_f32.argtypes = [P, S, P, P, P, S, S, ctypes.POINTER(_StepParams)]
-        L.orc_calc_lines_f64.argtypes = [P, S, P, P, P, S, S, ctypes.POINTER(_StepParams)]
+        L.orc_calc_lines_f32.argtypes = [P, S, P, P, P, S, S, ctypes.POINTER(_StepParams), P]
+        L.orc_calc_lines_f64.argtypes = [P, S, P, P, P, S, S, ctypes.POINTER(_StepParams), P]
         L.orc_field_f32.argtypes = [P, S, P, S, ctypes.c_float, P]
         L.orc_field_f64.argtypes = [P, S, P, S, ctypes.c_double, P]
         L.o64_field.argtypes = [P, S, P, S, ctypes.c_double, P]
         L.o64_field_jvp.argtypes = [P, S, P, S, ctypes.c_double, P, P]
-        L.o64_trace.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams), ctypes.c_double, P]
+        L.o64_trace.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams), ctypes.c_double, P, P]
         L.o64_adjoint.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams),
                                   ctypes.c_double, P, P, P]
+        L.o64_adjoint_path.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams),
+                                       ctypes.c_double, P, P, P, P, P]
+        L.o64_adjoint_path.restype = ctypes.c_int
         L.o64_min_dist_f32.argtypes = [P, S, P, S, P]
         L.o64_path_dist_f32.argtypes = [P, P, S, S, P, ctypes.c_longlong, P]
         for f in ("orc_calc_lines_f32", "orc_calc_lines_f64", "o64_trace", "o64_adjoint"):
@@ -102,7 +105,7 @@ def _ptr(a: np.ndarray) -> int:
 
 def _params(rule) -> _StepParams:
     return _StepParams(int(rule.rule), float(rule.resolution), float(rule.h0),
-                       float(rule.hmin), float(rule.hmax), float(rule.turn))
+                       float(rule.hmin), float(rule.hmax), float(rule.turn), int(rule.stop))
 
 
 def _rows_from_seeds(seeds: np.ndarray, length: int, dtype) -> np.ndarray:
@@ -112,15 +115,16 @@ def _rows_from_seeds(seeds: np.ndarray, length: int, dtype) -> np.ndarray:
     return rows
 
 
-def calc_lines(charges, seeds, length, rule, dtype=np.float32):
-    """Restated reference tracer.  Returns (rc, rows[3, len, n])."""
+def calc_lines(charges, seeds, length, rule, dtype=np.float32, with_steps=False):
+    """Restated reference tracer.  Returns (rc, rows[3, len, n]) [, steps[n]]."""
     charges = np.ascontiguousarray(charges, dtype=dtype)
     rows = _rows_from_seeds(np.asarray(seeds), length, dtype)
+    steps = np.zeros(rows.shape[2], dtype=np.uint32)
     f = lib().orc_calc_lines_f32 if dtype == np.float32 else lib().orc_calc_lines_f64
     sp = _params(rule)
     rc = f(_ptr(charges), charges.shape[0], _ptr(rows[0]), _ptr(rows[1]), _ptr(rows[2]),
-           rows.shape[2], length, ctypes.byref(sp))
-    return rc, rows
+           rows.shape[2], length, ctypes.byref(sp), _ptr(steps))
+    return (rc, rows, steps) if with_steps else (rc, rows)
 
 
 def ref_calc_lines(charges, seeds, length, resolution, use_curvature, dtype=np.float32):
@@ -185,19 +189,20 @@ def path_dist(test_rows, ref_rows, first, window=64):
     return out
 
 
-def trace64(charges, seeds, length, rule, scale=None):
-    """FP64 trajectory (PLAIN or ANGLE).  Returns rows[len, n, 3] (float64)."""
+def trace64(charges, seeds, length, rule, scale=None, with_steps=False):
+    """FP64 trajectory (PLAIN or ANGLE).  Returns rows[len, n, 3] (float64) [, steps]."""
     charges = np.ascontiguousarray(charges, dtype=np.float64)
     seeds = np.ascontiguousarray(seeds, dtype=np.float64)
     n = seeds.shape[0]
     rows = np.zeros((length, n, 3), dtype=np.float64)
+    steps = np.zeros(n, dtype=np.uint32)
     sp = _params(rule)
     sc = rule.field_scale if scale is None else scale
     rc = lib().o64_trace(_ptr(charges), charges.shape[0], _ptr(seeds), n, length,
-                         ctypes.byref(sp), sc, _ptr(rows))
+                         ctypes.byref(sp), sc, _ptr(rows), _ptr(steps))
     if rc:
         raise ValueError(f"o64_trace rc={rc}")
-    return rows
+    return (rows, steps) if with_steps else rows
 
 
 def adjoint64(charges, seeds, length, rule, G, scale=None):
@@ -216,4 +221,27 @@ def adjoint64(charges, seeds, length, rule, G, scale=None):
                            ctypes.byref(sp), sc, _ptr(G), _ptr(gch), _ptr(gseed))
     if rc:
         raise ValueError(f"o64_adjoint rc={rc}")
+    return gch, gseed
+
+
+def adjoint64_path(charges, seeds, length, rule, G, path_rows, path_steps, scale=None):
+    """FP64 reverse sweep linearised along a given trajectory.
+
+    ``path_rows``: (3, len, n) float32 (e.g. the GPU's rows); ``path_steps``:
+    (n,) step counts.  G as in adjoint64.  Returns (grad_charges, grad_seeds)."""
+    charges = np.ascontiguousarray(charges, dtype=np.float64)
+    seeds = np.ascontiguousarray(seeds, dtype=np.float64)
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    path = np.ascontiguousarray(np.asarray(path_rows, dtype=np.float32).transpose(1, 2, 0))
+    st = np.ascontiguousarray(path_steps, dtype=np.uint32)
+    n = seeds.shape[0]
+    gch = np.zeros_like(charges)
+    gseed = np.zeros_like(seeds)
+    sp = _params(rule)
+    sc = rule.field_scale if scale is None else scale
+    rc = lib().o64_adjoint_path(_ptr(charges), charges.shape[0], _ptr(seeds), n, length,
+                                ctypes.byref(sp), sc, _ptr(G), _ptr(gch), _ptr(gseed), _ptr(path),
+                                _ptr(st))
+    if rc:
+        raise ValueError(f"o64_adjoint_path rc={rc}")
     return gch, gseed

@@ -33,7 +33,13 @@ enum { O64_PLAIN = 0, O64_REF = 1, O64_ANGLE = 2 };
 typedef struct o64_step_params {
     int rule;
     double resolution, h0, hmin, hmax, turn;
+    int stop; /* angle rule: a line ends at the first step whose direction reverses */
 } o64_step_params;
+
+static int reversed(const o64_step_params* sp, int step, const double* u, const double* up)
+{
+    return sp->stop && step > 1 && u[0] * up[0] + u[1] * up[1] + u[2] * up[2] < 0;
+}
 
 static inline void neu_add(double* s, double* c, double v)
 {
@@ -132,7 +138,8 @@ static double step_rule(const o64_step_params* sp, int step, const double* E,
 
 /* rows: len*n*3, step-major, rows[(s*n + line)*3 + k]; row 0 = seeds. */
 int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
-              size_t len, const o64_step_params* sp, double scale, double* rows)
+              size_t len, const o64_step_params* sp, double scale, double* rows,
+              unsigned* steps)
 {
     if (!n) return 1;
     if (sp->rule == O64_REF) return 4; /* the REF rule is restated in restate.c */
@@ -143,19 +150,29 @@ int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
     for (long long line = 0; line < nl; ++line) {
         double x[3] = {seeds[3 * line], seeds[3 * line + 1], seeds[3 * line + 2]};
         double up[3] = {0, 0, 0}, hprev = sp->h0;
+        int alive = 1;
+        unsigned taken = 0;
         for (int k = 0; k < 3; ++k) rows[3 * line + k] = x[k];
         for (size_t s = 1; s < len; ++s) {
-            double E[3], u[3], L, c;
-            int br;
-            field64(ch, p, scale, x, E);
-            double h = step_rule(sp, (int)s, E, up, hprev, u, &L, &c, &br);
-            for (int k = 0; k < 3; ++k) {
-                x[k] = x[k] + h * u[k];
-                rows[(s * n + line) * 3 + k] = x[k];
-                up[k] = u[k];
+            if (alive) {
+                double E[3], u[3], L, c;
+                int br;
+                field64(ch, p, scale, x, E);
+                double h = step_rule(sp, (int)s, E, up, hprev, u, &L, &c, &br);
+                if (reversed(sp, (int)s, u, up)) {
+                    alive = 0;
+                } else {
+                    for (int k = 0; k < 3; ++k) {
+                        x[k] = x[k] + h * u[k];
+                        up[k] = u[k];
+                    }
+                    hprev = h;
+                    taken = (unsigned)s;
+                }
             }
-            hprev = h;
+            for (int k = 0; k < 3; ++k) rows[(s * n + line) * 3 + k] = x[k];
         }
+        if (steps) steps[line] = taken;
     }
     return 0;
 }
@@ -163,9 +180,10 @@ int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
 /* Reverse-mode derivative of  L = sum_{s=0}^{len-1} G_s . x_s  (G: len*n*3,
  * same layout as rows) w.r.t. charges (gch: p*4 = d/dx, d/dy, d/dz, d/dq) and
  * seeds (gseed: n*3). */
-int o64_adjoint(const double* ch, size_t p, const double* seeds, size_t n,
-                size_t len, const o64_step_params* sp, double scale,
-                const double* G, double* gch, double* gseed)
+static int adjoint_impl(const double* ch, size_t p, const double* seeds, size_t n,
+                        size_t len, const o64_step_params* sp, double scale,
+                        const double* G, double* gch, double* gseed,
+                        const float* path, const unsigned* path_steps)
 {
     if (!n) return 1;
     if (sp->rule == O64_REF) return 4;
@@ -187,11 +205,22 @@ int o64_adjoint(const double* ch, size_t p, const double* seeds, size_t n,
         for (long long line = 0; line < nl; ++line) {
             double x[3] = {seeds[3 * line], seeds[3 * line + 1], seeds[3 * line + 2]};
             double up[3] = {0, 0, 0}, hprev = sp->h0;
+            int alive = 1;
             for (size_t s = 1; s < len; ++s) {
                 double E[3], u[3];
+                if (path) /* linearise along the given trajectory */
+                    for (int k = 0; k < 3; ++k) x[k] = path[((s - 1) * n + line) * 3 + k];
                 for (int k = 0; k < 3; ++k) X[3 * s + k] = x[k];
+                Br[s] = 4; /* ended: x_s = x_{s-1} */
+                if (path && s > path_steps[line]) alive = 0;
+                if (!alive) continue;
                 field64(ch, p, scale, x, E);
                 double h = step_rule(sp, (int)s, E, up, hprev, u, &Lr[s], &Cs[s], &Br[s]);
+                if (reversed(sp, (int)s, u, up) && !path) {
+                    alive = 0;
+                    Br[s] = 4;
+                    continue;
+                }
                 Hs[s] = h;
                 for (int k = 0; k < 3; ++k) {
                     U[3 * s + k] = u[k];
@@ -204,6 +233,11 @@ int o64_adjoint(const double* ch, size_t p, const double* seeds, size_t n,
             for (size_t s = len - 1; s >= 1; --s) {
                 const double* u = U + 3 * s;
                 for (int k = 0; k < 3; ++k) Xb[k] += G[(s * n + line) * 3 + k];
+                if (Br[s] == 4) { /* the line had ended: identity, no field dependence */
+                    Ub[0] = Ub[1] = Ub[2] = 0;
+                    Hb = 0;
+                    continue;
+                }
                 Hb += Xb[0] * u[0] + Xb[1] * u[1] + Xb[2] * u[2];
                 for (int k = 0; k < 3; ++k) Ub[k] += Hs[s] * Xb[k];
                 double Upb[3] = {0, 0, 0}, hpb = 0;
@@ -308,4 +342,23 @@ void o64_path_dist_f32(const float* test, const float* ref, size_t len, size_t n
             out[j * n + l] = (float)sqrt(best);
         }
     }
+}
+
+int o64_adjoint(const double* ch, size_t p, const double* seeds, size_t n, size_t len,
+                const o64_step_params* sp, double scale, const double* G, double* gch,
+                double* gseed)
+{
+    return adjoint_impl(ch, p, seeds, n, len, sp, scale, G, gch, gseed, NULL, NULL);
+}
+
+/* The same reverse sweep, linearised along a given float trajectory (points
+ * [len][n][3], e.g. the GPU's rows) with its per-line step counts: the FP64
+ * derivative of the discrete path that trajectory took.  Isolates the
+ * arithmetic error of a float adjoint from differences between float and
+ * double trajectories. */
+int o64_adjoint_path(const double* ch, size_t p, const double* seeds, size_t n, size_t len,
+                     const o64_step_params* sp, double scale, const double* G, double* gch,
+                     double* gseed, const float* path, const unsigned* path_steps)
+{
+    return adjoint_impl(ch, p, seeds, n, len, sp, scale, G, gch, gseed, path, path_steps);
 }

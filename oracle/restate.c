@@ -33,7 +33,10 @@
  *   h_1 = h0,  h_s = max(hmin, c_s*hmax > turn*h_{s-1} ? turn*h_{s-1}/c_s : hmax)
  *   x_s = x_{s-1} + h_s*u_s
  * with the field summed by the 3-argument PartField in charge order and k = 1.
- * Its op order is shared bit-for-bit with the HIP step code.
+ * With stop = 1 a line ends at the first step s > 1 whose direction reverses
+ * (u_s . u_{s-1} < 0, the line went through a sink): it does not move at that
+ * step or later, and its remaining rows repeat the end point.  Its op order is
+ * shared bit-for-bit with the HIP step code.
  *
  * Parity pins for this file: tests/test_oracle.py (golden vectors written by
  * the compiled reference, closed forms for a single charge, scale and
@@ -52,6 +55,7 @@ typedef struct orc_step_params {
     int rule;          /* ORC_PLAIN / ORC_REF / ORC_ANGLE */
     double resolution; /* plain & ref: step = 1/resolution */
     double h0, hmin, hmax, turn; /* angle rule */
+    int stop;                    /* angle rule: end lines whose direction reverses */
 } orc_step_params;
 
 /* ---------------------------------------------------------------------------
@@ -99,7 +103,8 @@ static void field_sum_##SUF(const T* ch, size_t p, int two_arg, T k,          \
 }                                                                              \
                                                                                \
 int orc_calc_lines_##SUF(const T* charges, size_t p, T* x, T* y, T* z,        \
-                         size_t n, size_t len, const orc_step_params* sp)      \
+                         size_t n, size_t len, const orc_step_params* sp,      \
+                         uint32_t* steps)                                      \
 {                                                                              \
     const T res = (T)sp->resolution;                                           \
     if (!n) return 1;                                                          \
@@ -113,8 +118,11 @@ int orc_calc_lines_##SUF(const T* charges, size_t p, T* x, T* y, T* z,        \
         /* SSE path: prevAccum = seed point, never updated (207-351) */        \
         const T Px = px, Py = py, Pz = pz;                                     \
         T upx = 0, upy = 0, upz = 0, hprev = (T)sp->h0;                        \
+        int alive = 1;                                                         \
+        uint32_t taken = 0;                                                    \
         for (size_t step = 1; step < len; ++step) {                            \
             T ex, ey, ez;                                                      \
+            if (!alive) goto store_##SUF;                                      \
             if (sp->rule == ORC_PLAIN) {                                       \
                 /* CPU_Implement.cpp:79-92 */                                  \
                 field_sum_##SUF(charges, p, 1, 0, px, py, pz, &ex, &ey, &ez);  \
@@ -139,6 +147,11 @@ int orc_calc_lines_##SUF(const T* charges, size_t p, T* x, T* y, T* z,        \
                                 &ex, &ey, &ez);                                \
                 T len_e = SQRT(ex * ex + ey * ey + ez * ez);                   \
                 T ux = ex / len_e, uy = ey / len_e, uz = ez / len_e;           \
+                if (sp->stop && step > 1 &&                                    \
+                    ux * upx + uy * upy + uz * upz < (T)0) {                   \
+                    alive = 0;                                                 \
+                    goto store_##SUF;                                          \
+                }                                                              \
                 T h = (T)sp->h0;                                               \
                 if (step > 1) {                                                \
                     T dx = ux - upx, dy = uy - upy, dz = uz - upz;             \
@@ -150,10 +163,13 @@ int orc_calc_lines_##SUF(const T* charges, size_t p, T* x, T* y, T* z,        \
                 px = px + h * ux; py = py + h * uy; pz = pz + h * uz;          \
                 upx = ux; upy = uy; upz = uz; hprev = h;                       \
             }                                                                  \
+            taken = (uint32_t)step;                                            \
+        store_##SUF:                                                           \
             x[step * n + line] = px;                                           \
             y[step * n + line] = py;                                           \
             z[step * n + line] = pz;                                           \
         }                                                                      \
+        if (steps) steps[line] = taken;                                        \
     }                                                                          \
     return 0;                                                                  \
 }                                                                              \

@@ -69,11 +69,14 @@ def test_argument_errors_without_device():
     assert L.em_calc_lines_f32(P(ch), 4, P(x), P(y), P(z), 64, 1, 10.0, 1, None) == 3
     assert b"len" in L.em_last_error()
     st = _lib.em_step(7, 10.0, 0.1, 0.005, 0.5, 0.05)
-    assert L.em_trace_host_f32(P(ch), 4, P(x), P(y), P(z), 64, 10, ctypes.byref(st), None, None) == 16
+    assert L.em_trace_host_f32(P(ch), 4, P(x), P(y), P(z), 64, 10, ctypes.byref(st), None, None, None) == 16
     st = _lib.em_step(2, 10.0, 0.1, 0.6, 0.5, 0.05)  # hmin > hmax
-    assert L.em_trace_host_f32(P(ch), 4, P(x), P(y), P(z), 64, 10, ctypes.byref(st), None, None) == 16
+    assert L.em_trace_host_f32(P(ch), 4, P(x), P(y), P(z), 64, 10, ctypes.byref(st), None, None, None) == 16
+    st = _lib.em_step(0, 10.0, 0.1, 0.005, 0.5, 0.05, 1)  # stop is an angle-rule option
+    assert L.em_trace_host_f32(P(ch), 4, P(x), P(y), P(z), 64, 10, ctypes.byref(st), None, None, None) == 16
+    assert b"angle" in L.em_last_error()
     assert L.em_trace_host_f32(None, 4, P(x), P(y), P(z), 64, 10, ctypes.byref(
-        _lib.em_step(2, 10.0, 0.1, 0.005, 0.5, 0.05)), None, None) in (16, 19)
+        _lib.em_step(2, 10.0, 0.1, 0.005, 0.5, 0.05, 1)), None, None, None) in (16, 19)
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK),

@@ -86,19 +86,28 @@ def test_gpu_matches_reference_golden(gpu, name):
     assert rep["prefix_within_tol"] >= 0.95
 
 
-@pytest.mark.parametrize("rule", [rules.ref(10.0), rules.plain(10.0), rules.angle()],
-                         ids=["ref", "plain", "angle"])
+@pytest.mark.parametrize("rule", [rules.ref(10.0), rules.plain(10.0), rules.angle(),
+                                  rules.angle(stop=False)],
+                         ids=["ref", "plain", "angle", "angle-nostop"])
 def test_config0_lines_vs_cpu_path(gpu, orc, rule):
     """Config 0 charges (1,000 in [-10,10]^3), 512 of its 8,192 lines, 2,500 rows."""
     w = gen.config0()
     idx = np.arange(0, w.n, 16)
     seeds = w.seeds[idx]
-    g_rows = gpu_trace(gpu, w.charges, seeds, w.len, rule)
-    rc, c_rows = orc.calc_lines(w.charges, seeds, w.len, rule)
+    rows_t, steps_t = gpu.trace(dev(w.charges), dev(seeds), w.len, rule, with_steps=True)
+    torch.cuda.synchronize()
+    g_rows, g_steps = rows_t.cpu().numpy(), steps_t.cpu().numpy().astype(np.int64)
+    rc, c_rows, c_steps = orc.calc_lines(w.charges, seeds, w.len, rule, with_steps=True)
     assert rc == 0
     r64_rule = rules.plain(rule.resolution) if rule.rule == rules.REF else rule
     r64 = rows64_to_soa(orc.trace64(w.charges, seeds, w.len, r64_rule))
     assert np.isfinite(g_rows).all()
+    same = g_steps == c_steps.astype(np.int64)
+    print(f"[steps] config0/{rules.NAMES[rule.rule]}: identical step counts {same.sum()}/{len(same)}; "
+          f"mean steps {g_steps.mean():.1f} (stop={rule.stop})", flush=True)
+    if not rule.stop:
+        assert (g_steps == w.len - 1).all()
+    assert same.mean() >= 0.99
     check_against_cpu(f"config0/{rules.NAMES[rule.rule]}", g_rows, c_rows, r64, w.charges, rule)
 
 

@@ -111,6 +111,32 @@ def test_angle_rule_single_charge(orc):
     np.testing.assert_allclose(r64, r[:, None, None] * d[None], atol=1e-12)
 
 
+def test_angle_rule_stops_at_a_sink(orc):
+    """A line aimed at a lone sink moves straight in (h0, then hmax: chord 0),
+    overshoots it, and ends at the step whose direction reverses."""
+    c = np.array([[0.0, 0.0, 0.0, -1.0]], np.float64)
+    d = np.array([[0.6, 0.0, 0.8]])
+    seeds = 2.3 * d  # distance 2.3: 2.3 -> 2.2 -> 1.7 -> 1.2 -> 0.7 -> 0.2 -> -0.3 (past)
+    rule = rules.angle()
+    for dt in (np.float64, np.float32):
+        rc, rows, steps = orc.calc_lines(c.astype(dt), seeds.astype(dt), 12, rule, dtype=dt,
+                                         with_steps=True)
+        assert rc == 0
+        r = np.array([2.3, 2.2, 1.7, 1.2, 0.7, 0.2, -0.3])
+        # step 7 sees the field reversed: the line ends at x_6 and repeats it
+        assert int(steps[0]) == 6
+        expect = np.concatenate([r, np.full(5, -0.3)])[:, None] * d
+        tol = 1e-12 if dt == np.float64 else 2e-6
+        np.testing.assert_allclose(rows[:, :, 0].T, expect, atol=tol)
+    r64, s64 = orc.trace64(c, seeds, 12, rule, with_steps=True)
+    assert int(s64[0]) == 6
+    np.testing.assert_allclose(r64[:, 0, :], expect, atol=1e-12)
+    # without the stop the line keeps bouncing across the sink
+    rc, rows, steps = orc.calc_lines(c, seeds, 12, rules.angle(stop=False), dtype=np.float64,
+                                     with_steps=True)
+    assert int(steps[0]) == 11 and not np.allclose(rows[:, 7:, 0], rows[:, 6:7, 0])
+
+
 def test_ref_rule_reduces_to_plain(orc):
     """k = |(E-P)xP|/|E|^3 < 2^-53 with k_e inside: (k+1) == 1 (CPU_Implement.cpp:327-331)."""
     w = gen.plain_config(64, 64, 10.0, 100, seed=2)
@@ -189,6 +215,20 @@ def test_adjoint64_matches_finite_differences(orc, rule):
             sm[i, k] -= h
             fds[i, k] = (loss(ch, sp) - loss(ch, sm)) / (2 * h)
     np.testing.assert_allclose(gseed, fds, rtol=1e-5, atol=1e-6 * np.abs(fds).max())
+
+
+@pytest.mark.parametrize("rule", [rules.plain(10.0), rules.angle()], ids=["plain", "angle"])
+def test_adjoint64_path_is_the_adjoint_along_a_given_path(orc, rule):
+    """Fed the FP64 trajectory itself, the path form reproduces o64_adjoint
+    (up to the float rounding of the stored path)."""
+    w = gen.plain_config(40, 32, 4.0, 60, seed=21)
+    r64, s64 = orc.trace64(w.charges, w.seeds, w.len, rule, with_steps=True)
+    G = np.random.default_rng(0).normal(size=r64.shape)
+    a, sa = orc.adjoint64(w.charges, w.seeds, w.len, rule, G)
+    b, sb = orc.adjoint64_path(w.charges, w.seeds, w.len, rule, G,
+                               rows64_to_soa(r64).astype(np.float32), s64)
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 2e-3
+    assert np.linalg.norm(sa - sb) / np.linalg.norm(sa) < 2e-3
 
 
 def test_restated_angle_f32_tracks_fp64(orc):
