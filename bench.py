@@ -1,23 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 field-line tracer.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1|config0|config2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-A "step" is one full trace of the workload: every line advanced len-1 steps
-through the O(p) Coulomb sum (charge-point interactions = p * n * (len-1)).
-Default workload: 16,384 charges in [-50,50]^3, 65,536 lines, 2,500 rows,
-curvature-adaptive (angle) step in [0.005, 0.5] -- inputs resident in HBM
-before the timed region.  Prints one JSON line.
+A "step" is one full trace of the headline workload: 16,384 charges in
+[-50,50]^3, 65,536 lines, 2,500 rows, curvature-adaptive (angle) step in
+[0.005, 0.5], every line advanced all 2,499 steps (charge-point interactions
+= p * n * (len-1) = 2.683e12 per step).  Inputs are resident in HBM before the
+timed region; the kernel time comes from HIP events on the stream the kernel
+runs on.  Prints one JSON line.
 
-The roofline figure uses 18 credited FLOP per interaction (15 for the Coulomb
-term + 3 to accumulate, the accounting of the reference's
-include/Electrostatics.h:62 and ElectroMag/src/CPU_Implement.cpp:32) against
-MI355X's 157.3 TFLOP/s FP32 peak; the kernel time comes from HIP events
-recorded on the stream the kernel runs on.
+Roofline: 18 credited FLOP per interaction (15 for the Coulomb term + 3 to
+accumulate, the reference's accounting: include/Electrostatics.h:62,
+ElectroMag/src/CPU_Implement.cpp:32) against MI355X's 157.3 TFLOP/s FP32 peak.
+The CPU baseline times the reference's own CalcField_CPU<float> (compiled from
+source into oracle/_ref), or the C restatement in oracle/ where that build is
+absent, on a bounded slice of the same workload.
 
-The CPU baseline times the reference's own CalcField_CPU<float> (compiled
-from source into oracle/_ref) -- or, where that build is absent, the C
-restatement in oracle/ -- on a bounded slice of the same workload.
+Extra fields (skipped with --no-extras): the same workload with line ends at
+sinks (useful interactions only), the batched workload (512 configurations,
+one hipGraph-replayed persistent launch) and the differentiation workload
+(forward + adjoint time over forward time).
 """
 from __future__ import annotations
 
@@ -40,33 +43,56 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=["config1", "config0", "config2"])
-    ap.add_argument("--rule", default="angle", choices=["angle", "ref", "plain"])
+    ap.add_argument("--workload", default="config1", choices=["config1", "config0"])
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="use the library's heuristic shape")
     ap.add_argument("--cpu-lines", type=int, default=1024,
                     help="lines of the workload timed on the CPU (multiple of 64)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run a 1 GiB device copy (calibrates the FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(w, rule, lines):
+def timed(fn, steps, warmup):
+    """Run fn warmup times, then time `steps` runs: (wall s, [kernel s each])."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, [a.elapsed_time(b) * 1e-3 for a, b in evs]
+
+
+def cpu_baseline(w, lines):
     """Time the reference (or the restatement) on `lines` lines of the workload."""
     import numpy as np
     sys.path.insert(0, ROOT)
     import oracle  # the checker / CPU baseline only
+    from emtrace import rules
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     lines = max(64, lines // 64 * 64)
     seeds = w.seeds[np.linspace(0, w.n - 1, lines).astype(np.int64)]
-    res = 1.0 / rule.h0 if rule.rule == 2 else rule.resolution
     t0 = time.perf_counter()
     if oracle.ref_lib() is not None:
-        rc, _, t_ref, _ = oracle.ref_calc_lines(w.charges, seeds, w.len, res, 1)
+        rc, _, _, _ = oracle.ref_calc_lines(w.charges, seeds, w.len, 10.0, 1)
         kind = "reference"
-        path = "CalcField_CPU<float>, useCurvature=true (SSE + OpenMP)"
+        path = "reference CalcField_CPU<float>, useCurvature=true (SSE + OpenMP)"
     else:
-        rc, _ = oracle.calc_lines(w.charges, seeds, w.len, oracle_rule(rule))
+        rc, _ = oracle.calc_lines(w.charges, seeds, w.len, rules.ref(10.0))
         kind = "port"
         path = "oracle/restate.c (scalar C + OpenMP)"
     dt = time.perf_counter() - t0
@@ -74,71 +100,63 @@ def cpu_baseline(w, rule, lines):
         return None
     inter = w.p * lines * (w.len - 1)
     return {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": kind,
-            "sample": f"{path}: {w.p} charges x {lines} of {w.n} lines x {w.len} rows, "
-                      f"step 1/{res:g}, {dt:.2f} s wall; cost is linear in the line count",
-            "seconds": dt}
+            "sample": f"{path}: {w.p} charges x {lines} of {w.n} lines x {w.len} rows "
+                      f"(every line all {w.len - 1} steps, step 0.1), {dt:.2f} s wall; cost is "
+                      f"linear in the line count", "seconds": dt}
 
 
-def oracle_rule(rule):
-    return rule
+def copy_only():
+    import torch
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4}), flush=True)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
-    import numpy as np
     import torch
     import emtrace
-    from emtrace import gen, rules
+    from emtrace import autotune, gen, rules
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     torch.cuda.init()
-    L = emtrace.lib()
-    emtrace._lib.check(L.em_init(0), "em_init")
-    rule = {"angle": rules.angle(), "ref": rules.ref(10.0), "plain": rules.plain(10.0)}[args.rule]
-    launch = emtrace.make_launch(args.lanes, args.tile, args.block)
+    emtrace._lib.check(emtrace.lib().em_init(0), "em_init")
+    if args.copy_only:
+        return copy_only()
 
-    if args.workload == "config2":
-        return bench_batched(args, rule, launch)
-
+    rule = rules.angle(stop=False)
     w = gen.config1() if args.workload == "config1" else gen.config0()
     charges = torch.from_numpy(w.charges).cuda()
     seeds = torch.from_numpy(w.seeds).cuda()
+    if args.lanes or args.tile or args.block:
+        launch, tuned = emtrace.make_launch(args.lanes, args.tile, args.block), "given"
+    elif args.no_tune:
+        launch, tuned = None, "heuristic"
+    else:
+        launch, report = autotune.tune_plain(charges, seeds, rule)
+        tuned = "autotuned"
     rows = emtrace.seed_rows(seeds, w.len)
+    steps_buf = torch.empty(w.n, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
 
-    def step():
-        emtrace.trace_into(charges, rows, w.n, rule, launch)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for a, b in evs:
-        a.record(stream)
-        step()
-        b.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern = [a.elapsed_time(b) * 1e-3 for a, b in evs]
+    wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule, launch, steps=steps_buf),
+                       args.steps, args.warmup)
     k_avg = sum(kern) / len(kern)
     inter = float(w.interactions)
     value = inter * args.steps / wall
-    achieved_tflops = inter * FLOP_PER_INTERACTION / k_avg / 1e12
-    ok = bool(torch.isfinite(rows).all().item())
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(w, rule, args.cpu_lines)
+    achieved = inter * FLOP_PER_INTERACTION / k_avg / 1e12
+    finite = bool(torch.isfinite(rows).all().item())
+    all_steps = bool((steps_buf == w.len - 1).all().item())
+    shape = (launch.lanes_per_line, launch.tile, launch.block) if launch is not None else None
 
     out = {
         "metric": METRIC,
@@ -153,56 +171,85 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: plain method, {w.p} charges, {w.n} lines, "
-                               f"{w.len} rows, {rules.NAMES[rule.rule]} step rule",
-                   "charges": w.p, "lines": w.n, "rows": w.len, "step_rule": rules.NAMES[rule.rule],
+        "config": {"workload": f"{args.workload}: plain method, {w.p} charges in "
+                               f"[-{50 if args.workload == 'config1' else 10}, ...]^3, {w.n} lines, "
+                               f"{w.len} rows, curvature-adaptive (angle) step in [0.005, 0.5], "
+                               f"every line all {w.len - 1} steps",
+                   "charges": w.p, "lines": w.n, "rows": w.len, "step_rule": "angle",
+                   "launch": {"lanes_per_line": shape[0], "tile": shape[1], "block": shape[2],
+                              "source": tuned} if shape else {"source": tuned},
                    "parallelism": "single GPU"},
         "line_steps_per_s": w.n * (w.len - 1) * args.steps / wall,
         "frac_fp32_peak": value * FLOP_PER_INTERACTION / (PEAK_FP32_TFLOPS * 1e12),
-        "finite_output": ok,
-        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-                     "traffic": None, "kernel_ms": k_avg * 1e3,
-                     "flop_per_launch": inter * FLOP_PER_INTERACTION},
-        "cpu_baseline": cpu,
+        "finite_output": finite,
+        "all_lines_full_length": all_steps,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel_ms": k_avg * 1e3, "flop_per_launch": inter * FLOP_PER_INTERACTION,
+                     "algorithmic_bytes_per_launch": 12.0 * w.n * (w.len - 1)},
     }
+    if not args.no_extras:
+        out["extras"] = extras(w, charges, seeds, value, launch)
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(w, args.cpu_lines)
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(args, rule, launch):
+def extras(w, charges, seeds, plain_value, launch):
     import torch
     import emtrace
-    from emtrace import gen
+    from emtrace import autotune, gen, rules
+    res = {}
+    # 1. same workload, lines end where they reverse (useful interactions only)
+    rule_s = rules.angle(stop=True)
+    rows = emtrace.seed_rows(seeds, w.len)
+    st = torch.empty(w.n, dtype=torch.int32, device="cuda")
+    wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule_s, launch, steps=st), 1, 1)
+    taken = float(st.double().sum().item())
+    res["config1_with_line_ends"] = {
+        "useful_interactions_per_s": w.p * taken / kern[0], "kernel_ms": kern[0] * 1e3,
+        "mean_steps": taken / w.n, "note": "lines end at sinks; waves whose lines all ended stop "
+        "evaluating, lines that ended inside a live wave idle their lanes"}
+    del rows
+    # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
     ws = gen.config2()
-    layout, charges, lines = emtrace.pack_batch(ws)
-    plan = emtrace.BatchPlan(layout, rule, launch, use_graph=True)
+    rule_b = rules.angle(stop=False)
+    blaunch, _ = autotune.tune_batch(ws, rule_b)
+    layout, bch, blines = emtrace.pack_batch(ws)
+    plan = emtrace.BatchPlan(layout, rule_b, blaunch, use_graph=True)
+    wall, kern = timed(lambda: plan.run(bch, blines), 2, 1)
+    binter = float(layout.interactions())
+    bval = binter * 2 / wall
     info = plan.info()
-    for _ in range(args.warmup):
-        plan.run(charges, lines)
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for a, b in evs:
-        a.record(stream)
-        plan.run(charges, lines)
-        b.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    k_avg = sum(a.elapsed_time(b) for a, b in evs) / len(evs) * 1e-3
-    inter = float(layout.interactions())
-    value = inter * args.steps / wall
-    out = {"metric": "batched charge-point interactions/s (config 2)", "value": value,
-           "unit": "interactions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"config2: {len(ws)} configurations, {int(layout.p.sum())} charges, "
-                                  f"{int(layout.n.sum())} lines, 1000 rows, hipGraph persistent launch",
-                      "plan": info},
-           "frac_fp32_peak": value * FLOP_PER_INTERACTION / (PEAK_FP32_TFLOPS * 1e12),
-           "kernel_ms": k_avg * 1e3}
-    print(json.dumps(out), flush=True)
+    res["config2_batched"] = {
+        "interactions": binter, "interactions_per_s": bval, "kernel_ms": min(kern) * 1e3,
+        "ratio_to_plain": bval / plain_value, "configs": len(ws), "items": info["items"],
+        "grid": info["grid"], "launch": {k: info[k] for k in ("lanes_per_line", "tile", "block")}}
+    plan.close()
+    del blines, bch
+    # 3. differentiation workload: forward vs forward + adjoint
+    w3 = gen.config3()
+    tq = gen.perturbed(w3.charges, 0.05)
+    c3 = torch.from_numpy(w3.charges).cuda()
+    s3 = torch.from_numpy(w3.seeds).cuda()
+    rule_d = rules.angle()
+    tgt = emtrace.trace(torch.from_numpy(tq).cuda(), s3, w3.len, rule_d)
+    every = 50
+    rows3 = emtrace.seed_rows(s3, w3.len)
+    ck = torch.empty(emtrace.ckpt_size(w3.n, w3.len, every), dtype=torch.float32, device="cuda")
+
+    def fwd():
+        emtrace.trace_into(c3, rows3, w3.n, rule_d, None, ck, every)
+
+    def fwd_adj():
+        fwd()
+        g = (2.0 / (w3.n * w3.len)) * (rows3 - tgt)
+        emtrace.adjoint(c3, rows3, ck, every, g, rule_d)
+
+    _, kf = timed(fwd, 5, 2)
+    _, kfa = timed(fwd_adj, 5, 2)
+    res["config3_adjoint"] = {"forward_ms": min(kf) * 1e3, "forward_adjoint_ms": min(kfa) * 1e3,
+                              "ratio": min(kfa) / min(kf), "ckpt_every": every}
+    return res
 
 
 if __name__ == "__main__":

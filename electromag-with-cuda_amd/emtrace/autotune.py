@@ -1,0 +1,177 @@
+"""Launch-shape autotuner for the tracer.
+
+A launch shape is (lanes per line S, threads per workgroup, charges per LDS
+tile).  The tuner times each candidate on the caller's own inputs with a
+reduced row count (the per-step cost is what differs between shapes) and
+keeps the fastest.  Results are cached in a JSON file keyed by device,
+problem size bucket and step rule (``$EMTRACE_TUNE_CACHE``, default
+``~/.cache/emtrace/tune.json``).
+
+Candidates are generated from the device's LDS budget: for each S in
+{1, 2, 4, 8, 16, 32} that gives at least one wave per SIMD, blocks of 256 and
+512 threads, and tiles that hold all charges when they fit, else the largest
+tile that leaves room for one or two resident workgroups per CU.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from typing import Dict, List, Optional, Tuple
+
+from . import api
+from .rules import StepRule
+
+LDS_BYTES = 160 * 1024
+GROUP = 16  # charges per lane per pipelined pair: tiles are multiples of S * GROUP
+
+
+def _cache_path() -> str:
+    return os.environ.get("EMTRACE_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "emtrace", "tune.json"))
+
+
+def _load() -> Dict[str, list]:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key: str, value: list) -> None:
+    data = _load()
+    data[key] = value
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache
+
+
+def _device_key() -> str:
+    import torch
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return f"{getattr(p, 'gcnArchName', p.name)}:{p.multi_processor_count}"
+
+
+def _bucket(v: int) -> int:
+    return 1 << max(0, int(math.ceil(math.log2(max(1, v)))))
+
+
+def _round_up(a: int, b: int) -> int:
+    return (a + b - 1) // b * b
+
+
+def candidates(p: int, n: int, cus: int) -> List[Tuple[int, int, int]]:
+    """(S, block, tile) shapes worth timing for p charges and n lines."""
+    out = []
+    for S in (1, 2, 4, 8, 16, 32):
+        waves = math.ceil(n * S / 64)
+        if S > 1 and waves > 16 * cus:
+            continue  # already many waves per SIMD: splitting the sum only adds reductions
+        if S > 1 and p / S < 32:
+            continue
+        for block in (256, 512):
+            q = max(256, S * GROUP)
+            need = _round_up(max(p, 1), q)
+            tiles = set()
+            for resident in (1, 2):
+                tmax = (LDS_BYTES // resident // 16) // q * q
+                if tmax < q:
+                    continue
+                if need <= tmax:
+                    tiles.add(need)
+                else:
+                    k = math.ceil(p / tmax)
+                    tiles.add(min(tmax, _round_up(math.ceil(p / k), q)))
+            for t in sorted(tiles):
+                out.append((S, block, t))
+    return out
+
+
+def _time(fn, reps: int = 3) -> float:
+    import torch
+    fn()  # warm-up (also JIT-free: kernels are precompiled)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b))
+    return best
+
+
+def tune_plain(charges, seeds, rule: StepRule, rows: int = 48, use_cache: bool = True,
+               max_candidates: Optional[int] = None):
+    """Best launch for tracing ``seeds`` through ``charges`` with ``rule``.
+
+    Returns (em_launch, report) where report lists (S, block, tile, ms)."""
+    import torch
+    p, n = int(charges.shape[0]), int(seeds.shape[0])
+    key = f"plain:{_device_key()}:{_bucket(p)}:{_bucket(n)}:{rule.rule}:{int(rule.stop)}"
+    if use_cache:
+        hit = _load().get(key)
+        if hit:
+            return api.make_launch(hit[0], hit[2], hit[1]), [tuple(hit) + ("cached",)]
+    cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    cands = candidates(p, n, cus)
+    if max_candidates:
+        cands = cands[:max_candidates]
+    buf = api.seed_rows(seeds, rows)
+    report = []
+    for S, block, tile in cands:
+        launch = api.make_launch(S, tile, block)
+        try:
+            ms = _time(lambda: api.trace_into(charges, buf, n, rule, launch))
+        except Exception as e:  # a shape the device cannot hold
+            report.append((S, block, tile, float("inf"), str(e)[:80]))
+            continue
+        report.append((S, block, tile, ms))
+    best = min(report, key=lambda r: r[3])
+    _store(key, list(best[:4]))
+    return api.make_launch(best[0], best[2], best[1]), report
+
+
+def tune_batch(workloads, rule: StepRule, rows: int = 48, use_cache: bool = True):
+    """Best launch for a batch plan over ``workloads`` (a list of gen.Workload):
+    each candidate plan is timed on the same configurations with ``rows`` rows."""
+    import torch
+    ps = [w.p for w in workloads]
+    ns = [w.n for w in workloads]
+    key = (f"batch:{_device_key()}:{len(workloads)}:{_bucket(max(ps))}:{_bucket(sum(ns))}:"
+           f"{rule.rule}:{int(rule.stop)}")
+    if use_cache:
+        hit = _load().get(key)
+        if hit:
+            return api.make_launch(hit[0], hit[2], hit[1]), [tuple(hit) + ("cached",)]
+    cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    short = [type(w)(w.charges, w.seeds, rows) for w in workloads]
+    layout, charges, lines = api.pack_batch(short)
+    report = []
+    seen = set()
+    for S, block, tile in candidates(max(ps), sum(ns), cus):
+        # persistent grid: several workgroups per CU keep every SIMD busy
+        for t in sorted({tile, min(tile, _round_up(max(ps), max(256, S * GROUP)))}):
+            if (S, block, t) in seen:
+                continue
+            seen.add((S, block, t))
+            try:
+                plan = api.BatchPlan(layout, rule, api.make_launch(S, t, block), use_graph=False)
+                ms = _time(lambda: plan.run(charges, lines))
+                plan.close()
+            except Exception as e:
+                report.append((S, block, t, float("inf"), str(e)[:80]))
+                continue
+            report.append((S, block, t, ms))
+    best = min(report, key=lambda r: r[3])
+    _store(key, list(best[:4]))
+    return api.make_launch(best[0], best[2], best[1]), report

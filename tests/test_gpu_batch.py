@@ -93,3 +93,23 @@ def test_full_size_config1_properties(gpu):
     gpu.trace_into(c, rows, w.n, rules.angle())
     torch.cuda.synchronize()
     assert torch.equal(rows[:, -1, :], a)
+
+
+def test_autotuner_picks_a_valid_shape(gpu, tmp_path, monkeypatch):
+    """The tuned shape is one of the candidates, is cached, and gives the same
+    lines as an explicit launch with that shape."""
+    from emtrace import autotune
+    monkeypatch.setenv("EMTRACE_TUNE_CACHE", str(tmp_path / "tune.json"))
+    w = gen.plain_config(1500, 3000, 10.0, 64, seed=9)
+    c = torch.from_numpy(w.charges).cuda()
+    s = torch.from_numpy(w.seeds).cuda()
+    launch, report = autotune.tune_plain(c, s, rules.angle(), rows=16)
+    shape = (launch.lanes_per_line, launch.block, launch.tile)
+    assert shape in [r[:3] for r in report] and all(np.isfinite(r[3]) for r in report)
+    again, rep2 = autotune.tune_plain(c, s, rules.angle(), rows=16)
+    assert (again.lanes_per_line, again.block, again.tile) == shape and rep2[0][-1] == "cached"
+    a = gpu.trace(c, s, w.len, rules.angle(), launch)
+    b = gpu.trace(c, s, w.len, rules.angle(), gpu.make_launch(shape[0], shape[2], shape[1]))
+    assert torch.equal(a, b)
+    blaunch, brep = autotune.tune_batch(_configs(count=6, seed=2, length=40), rules.angle(), rows=8)
+    assert blaunch.lanes_per_line >= 1 and any(np.isfinite(r[3]) for r in brep)
