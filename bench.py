@@ -188,10 +188,27 @@ def main():
                      "kernel_ms": k_avg * 1e3, "flop_per_launch": inter * FLOP_PER_INTERACTION,
                      "algorithmic_bytes_per_launch": 12.0 * w.n * (w.len - 1)},
     }
+    traffic = profiled_traffic(args.workload, shape)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_source"] = "profiles/trace_config1_hbm_traffic.md (rocprofv3 " \
+            "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, scales from a 1 GiB copy)"
     if not args.no_extras:
         out["extras"] = extras(w, charges, seeds, value, launch)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(w, args.cpu_lines)
     print(json.dumps(out), flush=True)
+
+
+def profiled_traffic(workload, shape):
+    """HBM bytes per launch from the committed counter profile, when this run
+    used the launch shape that was profiled (lanes 4, tile 4096, block 512)."""
+    path = os.path.join(ROOT, "profiles", "trace_config1_hbm_traffic.md")
+    if workload != "config1" or shape != (4, 4096, 512) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if "traffic_bytes_per_launch:" in line:
+            return float(line.split(":")[1].strip(" *\n"))
+    return None
 
 
 def extras(w, charges, seeds, plain_value, launch):
