@@ -266,6 +266,30 @@ def extras(w, charges, seeds, plain_value, launch):
     _, kfa = timed(fwd_adj, 5, 2)
     res["config3_adjoint"] = {"forward_ms": min(kf) * 1e3, "forward_adjoint_ms": min(kfa) * 1e3,
                               "ratio": min(kfa) / min(kf), "ckpt_every": every}
+    del rows3, ck, tgt
+    # 4. inverse fit: 64 configurations, 800 rows, 200 Adam iterations, each one
+    #    captured batched forward + adjoint + update
+    from emtrace.fit import InverseFit
+    ws4 = gen.config4()
+    truth = [gen.perturbed(w4.charges, 0.1, seed=i) for i, w4 in enumerate(ws4)]
+    fit = InverseFit(ws4, truth, rules.angle(), ckpt_every=50)
+    fit.plan.run(fit.theta, fit.lines)
+    loss0 = fit.loss()
+    fit.step()  # eager warm-up iteration + graph capture
+    torch.cuda.synchronize()
+    iters = 199
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        fit.step()
+    torch.cuda.synchronize()
+    per_iter = (time.perf_counter() - t0) / iters
+    fit.plan.run(fit.theta, fit.lines)
+    res["config4_inverse_fit"] = {
+        "configs": len(ws4), "charges": int(fit.layout.p.sum()), "lines": int(fit.layout.n.sum()),
+        "iterations": iters + 1, "ms_per_iteration": per_iter * 1e3, "loss_start": loss0,
+        "loss_end": fit.loss(), "recovery": fit.recovery(),
+        "note": "iteration = batched forward with checkpoints + dL/dx + batched adjoint + Adam, "
+                "replayed from one captured graph"}
     return res
 
 

@@ -7,10 +7,11 @@ keeps the fastest.  Results are cached in a JSON file keyed by device,
 problem size bucket and step rule (``$EMTRACE_TUNE_CACHE``, default
 ``~/.cache/emtrace/tune.json``).
 
-Candidates are generated from the device's LDS budget: for each S in
-{1, 2, 4, 8, 16, 32} that gives at least one wave per SIMD, blocks of 256 and
-512 threads, and tiles that hold all charges when they fit, else the largest
-tile that leaves room for one or two resident workgroups per CU.
+Candidates are generated from the device's LDS budget: lanes per line S in
+{1, 2, ..., 32} (single configuration: until the grid holds 8 waves per SIMD;
+batch: 1-8), blocks of 256 and 512 threads, and tiles that hold all charges
+when they fit, else the largest tile that leaves room for one or two resident
+workgroups per CU.
 """
 from __future__ import annotations
 
@@ -67,13 +68,19 @@ def _round_up(a: int, b: int) -> int:
     return (a + b - 1) // b * b
 
 
-def candidates(p: int, n: int, cus: int) -> List[Tuple[int, int, int]]:
-    """(S, block, tile) shapes worth timing for p charges and n lines."""
+def candidates(p: int, n: int, cus: int, batch: bool = False) -> List[Tuple[int, int, int]]:
+    """(S, block, tile) shapes worth timing for p charges and n lines.
+
+    Single configuration: S grows until the grid has 8 waves per SIMD.  Batch:
+    S in {1, 2, 4, 8} always -- a smaller line block per work item evens out
+    the persistent grid's finishing times even when waves are plentiful."""
     out = []
     for S in (1, 2, 4, 8, 16, 32):
-        waves = math.ceil(n * S / 64)
-        if S > 1 and waves > 16 * cus:
-            continue  # already many waves per SIMD: splitting the sum only adds reductions
+        if batch:
+            if S > 8:
+                continue
+        elif S > 1 and math.ceil(n * (S // 2) / 64) >= 8 * 4 * cus:
+            continue  # already 8 waves per SIMD at S/2: splitting further only adds reductions
         if S > 1 and p / S < 32:
             continue
         for block in (256, 512):
@@ -158,7 +165,7 @@ def tune_batch(workloads, rule: StepRule, rows: int = 48, use_cache: bool = True
     layout, charges, lines = api.pack_batch(short)
     report = []
     seen = set()
-    for S, block, tile in candidates(max(ps), sum(ns), cus):
+    for S, block, tile in candidates(max(ps), sum(ns), cus, batch=True):
         # persistent grid: several workgroups per CU keep every SIMD busy
         for t in sorted({tile, min(tile, _round_up(max(ps), max(256, S * GROUP)))}):
             if (S, block, t) in seen:

@@ -1,0 +1,111 @@
+"""Inverse fit of charge configurations to traced field lines.
+
+Given many configurations (a BatchLayout of charges and seeds) and target
+lines (traced from the true charges), minimise
+
+    L = sum_c mean_{lines, rows of c} |x - x_target|^2
+
+over every charge's position and magnitude with Adam.  One iteration is a
+batched forward with checkpoints, the loss gradient dL/dx, the batched adjoint
+(dL/d charges) and an elementwise Adam update, all on one stream.  After one
+eager warm-up iteration (which sizes every work buffer), the iteration is
+captured into a single graph and replayed; the graph holds no memset and no
+reduction (the loss value is computed outside it).  All kernels are
+deterministic, so replays equal eager iterations bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import api
+from .rules import StepRule, angle
+
+
+class InverseFit:
+    def __init__(self, workloads, target_charges, rule: StepRule = angle(), ckpt_every: int = 50,
+                 lr_pos: float = 5e-3, lr_q: float = 2.5e-3, betas=(0.9, 0.999), eps: float = 1e-12,
+                 launch=None):
+        import torch
+        self.torch = torch
+        self.rule = rule
+        self.every = ckpt_every
+        self.layout, charges, self.lines = api.pack_batch(workloads)
+        self.theta = charges.clone()                       # parameters (sum p, 4)
+        self.theta0 = charges.clone()
+        self.true = torch.from_numpy(np.concatenate(target_charges)).cuda()
+        self.plan = api.BatchPlan(self.layout, rule, launch, use_graph=False)
+        # targets: the same seeds traced through the true charges
+        self.target = self.lines.clone()
+        self.plan.run(self.true, self.target)
+        self.ckpt = torch.empty(max(1, self.plan.ckpt_size(ckpt_every)), dtype=torch.float32,
+                                device="cuda")
+        self.grad = torch.zeros_like(self.theta)
+        self.G = torch.zeros_like(self.lines)
+        # per-element weight 2 / (n_c * len_c): d/dx of each configuration's mean
+        wts = np.empty(self.layout.total_line_elems, dtype=np.float32)
+        for i in range(len(self.layout.p)):
+            o, n, ln = int(self.layout.line_off[i]), int(self.layout.n[i]), int(self.layout.len[i])
+            wts[o:o + n * ln] = 2.0 / (n * ln)
+        self.wts = torch.from_numpy(wts).cuda()
+        self.m = torch.zeros_like(self.theta)
+        self.v = torch.zeros_like(self.theta)
+        self.t = torch.zeros((), dtype=torch.float32, device="cuda")
+        self.lr = torch.tensor([lr_pos, lr_pos, lr_pos, lr_q], dtype=torch.float32, device="cuda")
+        self.b1, self.b2, self.eps = float(betas[0]), float(betas[1]), float(eps)
+        self.graph = None
+
+    # one iteration, as a sequence of launches on the current stream
+    def _body(self):
+        torch = self.torch
+        self.plan.run_ckpt(self.theta, self.lines, self.ckpt, self.every)
+        torch.sub(self.lines, self.target, out=self.G)
+        self.G.mul_(self.wts)
+        self.plan.adjoint(self.theta, self.lines, self.ckpt, self.every, self.G, self.grad)
+        self.t.add_(1.0)
+        self.m.mul_(self.b1).add_(self.grad, alpha=1.0 - self.b1)
+        self.v.mul_(self.b2).addcmul_(self.grad, self.grad, value=1.0 - self.b2)
+        bc1 = 1.0 - torch.pow(torch.full_like(self.t, self.b1), self.t)
+        bc2 = 1.0 - torch.pow(torch.full_like(self.t, self.b2), self.t)
+        upd = (self.m / bc1) / ((self.v / bc2).sqrt() + self.eps)
+        self.theta.sub_(upd * self.lr)
+
+    def step_eager(self):
+        self._body()
+
+    def capture(self):
+        """Warm up once (allocates the adjoint's work lists), then capture."""
+        torch = self.torch
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._body()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body()
+        torch.cuda.synchronize()
+
+    def step(self):
+        if self.graph is None:
+            self.capture()
+        else:
+            self.graph.replay()
+
+    # ---- reporting (outside the graph) ----
+    def loss(self) -> float:
+        torch = self.torch
+        d = (self.lines - self.target) ** 2
+        d = d.sum(dim=0) * (self.wts / 2.0)
+        return float(d.sum().item())
+
+    def recovery(self):
+        """RMS position and magnitude errors vs the true charges, now and at start."""
+        torch = self.torch
+        def rms(a, b):
+            pos = torch.sqrt(((a[:, :3] - b[:, :3]) ** 2).sum(dim=1).mean()).item()
+            mag = torch.sqrt(((a[:, 3] - b[:, 3]) ** 2).mean()).item()
+            return pos, mag
+        p0, q0 = rms(self.theta0, self.true)
+        p1, q1 = rms(self.theta, self.true)
+        return {"pos_rms_start": p0, "pos_rms": p1, "q_rms_start": q0, "q_rms": q1}

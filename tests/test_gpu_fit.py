@@ -1,0 +1,113 @@
+"""Inverse fit (batched forward + adjoint + Adam in one captured graph).
+
+* graph replays equal eager iterations bit for bit, with eager work and
+  synchronisations between replays;
+* the batched adjoint's gradients on a 4-configuration subset of the config-4
+  workload agree with the FP64 oracle: the FP64 sweep along the GPU's own path
+  (all lines), and the FP64 adjoint of the FP64 trajectory on the lines that
+  follow it;
+* a short fit lowers the loss.
+"""
+import numpy as np
+import pytest
+import torch
+
+from emtrace import gen, rules
+from emtrace.fit import InverseFit
+from helpers import rows64_to_soa
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(count, p_range, n_range, length, sigma, seed):
+    ws = gen.batched_configs(count, p_range, n_range, length, seed)
+    truth = [gen.perturbed(w.charges, sigma, seed=seed * 100 + i) for i, w in enumerate(ws)]
+    return ws, truth
+
+
+def test_graph_replay_equals_eager_iterations(gpu):
+    ws, truth = _problem(5, (64, 400), (64, 300), 60, 0.1, 3)
+    eager = InverseFit(ws, truth, rules.angle(), ckpt_every=9)
+    want = []
+    for _ in range(4):
+        eager.step_eager()
+        torch.cuda.synchronize()
+        want.append(eager.theta.clone())
+    graphed = InverseFit(ws, truth, rules.angle(), ckpt_every=9)
+    graphed.step()  # warm-up iteration (eager) + capture
+    torch.cuda.synchronize()
+    assert torch.equal(graphed.theta, want[0])
+    for k in range(1, 4):
+        graphed.step()  # replay
+        torch.cuda.synchronize()
+        _ = graphed.loss()  # eager work between replays
+        assert torch.equal(graphed.theta, want[k]), k
+
+
+def test_config4_subset_gradients_vs_fp64(gpu, orc):
+    """First 4 configurations of the config-4 workload (800 rows, sigma 0.1)."""
+    ws_all = gen.config4()
+    ws = ws_all[:4]
+    truth = [gen.perturbed(w.charges, 0.1, seed=i) for i, w in enumerate(ws)]
+    fit = InverseFit(ws, truth, rules.angle(), ckpt_every=50)
+    fit.plan.run_ckpt(fit.theta, fit.lines, fit.ckpt, fit.every)
+    torch.sub(fit.lines, fit.target, out=fit.G)
+    fit.G.mul_(fit.wts)
+    fit.plan.adjoint(fit.theta, fit.lines, fit.ckpt, fit.every, fit.G, fit.grad)
+    steps = torch.empty(int(fit.layout.n.sum()), dtype=torch.int32, device="cuda")
+    fit.plan.run(fit.theta, fit.lines, steps)
+    torch.cuda.synchronize()
+    grad = fit.grad.cpu().numpy().astype(np.float64)
+    lines = fit.lines
+    G = fit.G
+    steps = steps.cpu().numpy().astype(np.int64)
+    rep = []
+    off_line = 0
+    for i, w in enumerate(ws):
+        rows = fit.layout.rows_view(lines, i).cpu().numpy()
+        g_rows = fit.layout.rows_view(G, i).cpu().numpy().astype(np.float64)
+        st = steps[off_line:off_line + w.n]
+        off_line += w.n
+        G64 = np.ascontiguousarray(g_rows.transpose(1, 2, 0))
+        gp, _ = orc.adjoint64_path(w.charges, w.seeds, w.len, rules.angle(), G64, rows, st)
+        o = int(fit.layout.charge_off[i])
+        gg = grad[o:o + w.p]
+        r_path = np.linalg.norm(gg - gp) / np.linalg.norm(gp)
+        # own FP64 trajectory, on the lines that follow it
+        r64, s64 = orc.trace64(w.charges, w.seeds, w.len, rules.angle(), with_steps=True)
+        err = np.abs(rows.astype(np.float64) - rows64_to_soa(r64)).max(axis=(0, 1))
+        ok = (st == s64.astype(np.int64)) & (err <= 1e-3)
+        Gm = G64.copy()
+        Gm[:, ~ok, :] = 0.0
+        g64, _ = orc.adjoint64(w.charges, w.seeds, w.len, rules.angle(), Gm)
+        Gmf = np.ascontiguousarray(Gm.transpose(2, 0, 1)).astype(np.float32)
+        rep.append((w.p, w.n, float(r_path), float(ok.mean()), g64, Gmf, i))
+        print(f"[config4 subset] cfg {i}: p={w.p} n={w.n} rel(gpu, FP64 along gpu path) {r_path:.2e}; "
+              f"lines following FP64 {ok.mean():.3f}", flush=True)
+        assert r_path < 1e-3
+        assert ok.mean() > 0.8
+    # the masked gradients (agreeing lines only) through the batched GPU adjoint
+    for (p, n, _, _, g64, Gmf, i) in rep:
+        fit.layout.rows_view(fit.G, i).copy_(torch.from_numpy(Gmf).cuda())
+    fit.plan.adjoint(fit.theta, fit.lines, fit.ckpt, fit.every, fit.G, fit.grad)
+    torch.cuda.synchronize()
+    grad = fit.grad.cpu().numpy().astype(np.float64)
+    for (p, n, _, _, g64, _, i) in rep:
+        o = int(fit.layout.charge_off[i])
+        r = np.linalg.norm(grad[o:o + p] - g64) / np.linalg.norm(g64)
+        print(f"[config4 subset] cfg {i}: rel(gpu, FP64 adjoint of FP64 trajectory) {r:.2e}", flush=True)
+        assert r < 1e-1
+
+
+def test_short_fit_lowers_the_loss(gpu):
+    ws, truth = _problem(8, (128, 512), (256, 512), 200, 0.1, 5)
+    fit = InverseFit(ws, truth, rules.angle(), ckpt_every=50)
+    fit.plan.run(fit.theta, fit.lines)
+    l0 = fit.loss()
+    for _ in range(30):
+        fit.step()
+    fit.plan.run(fit.theta, fit.lines)
+    l1 = fit.loss()
+    rec = fit.recovery()
+    print(f"[fit] loss {l0:.4g} -> {l1:.4g}; recovery {rec}", flush=True)
+    assert l1 < l0
