@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "em_internal.hpp"
+#include "emtrace/legacy.hpp"
 
 namespace em {
 
@@ -395,6 +396,34 @@ int em_field_dev_f32(const float* d_charges, size_t p, const float* d_px, const 
     EM_TRY(choose_shape(dev, p, m, launch, sh));
     return launch_field(sh, reinterpret_cast<const float4*>(d_charges), (uint32_t)p, d_px, d_py,
                         d_pz, m, scale, d_ex, d_ey, d_ez, static_cast<hipStream_t>(stream));
+}
+
+int em_legacy_rand(uint32_t seed, int32_t* out, size_t count)
+{
+    clear_error();
+    if (count && !out) EM_FAIL(EM_ERR_INVALID, "NULL output");
+    emtrace::legacy::LibcRand g(seed);
+    for (size_t i = 0; i < count; ++i) out[i] = g.next();
+    return EM_OK;
+}
+
+int em_legacy_problem_f32(float* charges, size_t p, uint32_t charge_seed, float* x, float* y,
+                          float* z, size_t width, size_t height, size_t depth, int random_lines,
+                          uint32_t line_seed)
+{
+    clear_error();
+    const size_t n = width * height * depth;
+    if (p && !charges) EM_FAIL(EM_ERR_INVALID, "NULL charges");
+    if (n && (!x || !y || !z)) EM_FAIL(EM_ERR_INVALID, "NULL line arrays");
+    emtrace::legacy::LibcRand g(charge_seed);
+    emtrace::legacy::init_charges(charges, p, g);
+    if (random_lines) {
+        g.reseed(line_seed);
+        emtrace::legacy::init_random_lines(x, y, z, n, g);
+    } else {
+        emtrace::legacy::init_grid(x, y, z, width, height, depth);
+    }
+    return EM_OK;
 }
 
 }  // extern "C"

@@ -4,7 +4,7 @@ Host-side mirror of the reference's calc-lines interface over the C ABI of
 ``lib/libemtrace.so`` (include/emtrace.h).  The compute runs only in the HIP
 library; importing this package never falls back to a CPU path.
 """
-from . import gen, rules
+from . import gen, legacy, rules
 from ._lib import EmError, LibraryMissing, lib, LIB_PATH
 from .api import (BatchLayout, BatchPlan, calc_lines, ckpt_size, field, make_launch, make_step,
                   pack_batch, seed_rows, trace, trace_host, trace_into)
@@ -12,7 +12,7 @@ from .rules import ANGLE, PLAIN, REF, StepRule
 from .autograd import adjoint, batch_trace, trace_lines
 
 __all__ = [
-    "gen", "rules", "EmError", "LibraryMissing", "lib", "LIB_PATH", "BatchLayout", "BatchPlan",
+    "gen", "legacy", "rules", "EmError", "LibraryMissing", "lib", "LIB_PATH", "BatchLayout", "BatchPlan",
     "calc_lines", "ckpt_size", "field", "make_launch", "make_step", "pack_batch", "seed_rows",
     "trace", "trace_host", "trace_into", "ANGLE", "PLAIN", "REF", "StepRule", "adjoint",
     "batch_trace", "trace_lines",

@@ -96,6 +96,8 @@ SIGNATURES = {
     "em_batch_ckpt_floats": (SZ, [P, U32]),
     "em_batch_run_ckpt": (I, [P, P, P, P, P, P, U32, P, P]),
     "em_batch_adjoint": (I, [P, P, P, P, P, P, U32, P, P, P, P, P]),
+    "em_legacy_rand": (I, [U32, P, SZ]),
+    "em_legacy_problem_f32": (I, [P, SZ, U32, P, P, P, SZ, SZ, SZ, I, U32]),
 }
 
 _lib = None

@@ -179,6 +179,22 @@ int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x,
                      uint32_t ckpt_every, const float* d_gx, const float* d_gy,
                      const float* d_gz, float* d_grad_charges, void* stream);
 
+/* ---- the reference application's problem generator -----------------------
+ * Host-only.  Restates ElectroMag/src/Electromag utils.h:29-111 (libc rand()
+ * under srand) with a built-in copy of glibc's rand(), so the reference's
+ * problems are reproduced bit for bit on any host.
+ * em_legacy_rand: out[i] = the i-th rand() after srand(seed).
+ * em_legacy_problem_f32: InitializePointChargeArray (p records {x,y,z,q},
+ *   srand(charge_seed); the reference uses 1) followed by
+ *   InitializeFieldLineArray into x/y/z (n = width*height*depth seeds): the
+ *   grid 10*((-w/2 + 1e-5) + i), ... when random_lines == 0, else three
+ *   rand() draws per line after srand(line_seed). */
+int em_legacy_rand(uint32_t seed, int32_t* out, size_t count);
+int em_legacy_problem_f32(float* charges, size_t p, uint32_t charge_seed,
+                          float* x, float* y, float* z, size_t width,
+                          size_t height, size_t depth, int random_lines,
+                          uint32_t line_seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,8 +11,9 @@ Contents
   (ElectroMag/src/CPU_Implement.cpp:38-102, 207-351, 366-498) plus the
   project-defined angle rule in the same op order as the HIP step code.
 * ``oracle64.c`` -- FP64 field, field-gradient JVP, trajectory and adjoint.
-* ``ref_driver.cpp`` + ``Makefile ref`` -- the reference's own
-  CPU_Implement.cpp compiled from /root/reference into ``_ref/libemref.so``
+* ``ref_driver.cpp`` + ``ref_init_driver.cpp`` + ``Makefile ref`` -- the
+  reference's own CPU_Implement.cpp and the problem initialisation of its
+  Electromag utils.h compiled from /root/reference into ``_ref/libemref.so``
   (absent where the reference is absent; tests then use the golden vectors
   under tests/golden/, which that build wrote).
 
@@ -95,6 +96,10 @@ def ref_lib() -> Optional[ctypes.CDLL]:
         L.ref_calc_lines_f64.argtypes = [P, S, P, P, P, S, S, ctypes.c_double, ctypes.c_int, P, P]
         L.ref_calc_lines_f32.restype = ctypes.c_int
         L.ref_calc_lines_f64.restype = ctypes.c_int
+        L.ref_init_charges_f32.argtypes = [P, S, ctypes.c_int]
+        L.ref_init_charges_f32.restype = None
+        L.ref_init_lines_f32.argtypes = [P, P, P, S, S, S]
+        L.ref_init_lines_f32.restype = None
         _ref = L
     return _ref
 
@@ -140,6 +145,30 @@ def ref_calc_lines(charges, seeds, length, resolution, use_curvature, dtype=np.f
     rc = f(_ptr(charges), charges.shape[0], _ptr(rows[0]), _ptr(rows[1]), _ptr(rows[2]),
            rows.shape[2], length, resolution, int(use_curvature), ctypes.byref(t), ctypes.byref(g))
     return rc, rows, t.value, g.value
+
+
+def ref_init_charges(p: int) -> np.ndarray:
+    """The reference's InitializePointChargeArray<float>(charges, p, false)
+    (ElectroMag/src/Electromag utils.h:91-111: srand(1), four rand() draws per
+    charge).  Returns (p, 4) float32 {x, y, z, q}."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libemref.so not built (reference absent)")
+    out = np.empty((p, 4), np.float32)
+    L.ref_init_charges_f32(_ptr(out), p, 0)
+    return out
+
+
+def ref_init_grid(width: int, height: int, depth: int) -> np.ndarray:
+    """The reference's InitializeFieldLineArray<float>(..., random=false)
+    (Electromag utils.h:29-70).  Returns (n, 3) float32 seeds."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libemref.so not built (reference absent)")
+    n = width * height * depth
+    xyz = np.empty((3, n), np.float32)
+    L.ref_init_lines_f32(_ptr(xyz[0]), _ptr(xyz[1]), _ptr(xyz[2]), width, height, depth)
+    return np.ascontiguousarray(xyz.T)
 
 
 def field_ref_f32(charges, pts, k=np.float32(K_E)):

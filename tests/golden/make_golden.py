@@ -80,6 +80,38 @@ def main() -> None:
                         codes=np.array(list(codes.values()), dtype=np.int32),
                         args=np.array([ERROR_CASES[k][:4] for k in codes], dtype=np.float64))
     print("error codes:", codes)
+    legacy()
+
+
+# The reference application's own problems (ElectroMag.cpp:43-58 presets,
+# Electromag utils.h:29-111 initialisation), traced by the reference's
+# CalcField_CPU<float> with the application's arguments (resolution 1,
+# useCurvature true; ElectroMag.cpp:296, 305-306).  Lines are independent, so
+# a subset of a preset's seeds traces to the same rows as the full run.
+LEGACY_GRIDS = [(16, 16, 1), (5, 3, 4), (7, 1, 3), (128, 128, 1)]
+LEGACY_TRACES = {
+    # name: (nx, ny, nz, p, len, line stride, row stride)
+    "legacy_bogo_ref": (16, 16, 1, 50, 500, 4, 2),
+    "legacy_default_ref": (128, 128, 1, 1024, 2500, 256, 10),
+}
+
+
+def legacy() -> None:
+    out = {"charges_p1024": oracle.ref_init_charges(1024)}
+    for (w, h, d) in LEGACY_GRIDS:
+        out[f"grid_{w}x{h}x{d}"] = oracle.ref_init_grid(w, h, d)
+    np.savez_compressed(os.path.join(OUT, "legacy_init.npz"), **out)
+    print("legacy_init:", {k: v.shape for k, v in out.items()})
+    for name, (nx, ny, nz, p, length, ls, rs) in LEGACY_TRACES.items():
+        charges = oracle.ref_init_charges(p)
+        seeds = oracle.ref_init_grid(nx, ny, nz)[::ls]
+        assert seeds.shape[0] % 64 == 0  # the SSE path's line multiple
+        rc, rows, _, _ = oracle.ref_calc_lines(charges, seeds, length, 1.0, 1)
+        assert rc == 0, (name, rc)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), lines=np.arange(0, nx * ny * nz, ls),
+                            row_stride=np.array(rs), rows=rows[:, ::rs, :],
+                            preset=np.array([nx, ny, nz, p, length]))
+        print(f"{name}: lines {seeds.shape[0]}, rows {rows[:, ::rs, :].shape}")
 
 
 if __name__ == "__main__":
