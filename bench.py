@@ -19,8 +19,9 @@ absent, on a bounded slice of the same workload.
 
 Extra fields (skipped with --no-extras): the same workload with line ends at
 sinks (useful interactions only), the batched workload (512 configurations,
-one hipGraph-replayed persistent launch) and the differentiation workload
-(forward + adjoint time over forward time).
+one hipGraph-replayed persistent launch), the differentiation workload
+(forward + adjoint time over forward time), the inverse fit, the FP64 kernel
+and the reference application's default problem.
 """
 from __future__ import annotations
 
@@ -290,6 +291,38 @@ def extras(w, charges, seeds, plain_value, launch):
         "loss_end": fit.loss(), "recovery": fit.recovery(),
         "note": "iteration = batched forward with checkpoints + dL/dx + batched adjoint + Adam, "
                 "replayed from one captured graph"}
+    del fit
+    # 5. FP64 kernel (IEEE binary64, reference op order): config-1 charges and
+    #    lines, 250 rows, plain step
+    rows_n = 250
+    q64, s64 = charges.double(), seeds.double()
+    rule_p = rules.plain(10.0)
+    r64 = torch.empty((3, rows_n, w.n), dtype=torch.float64, device="cuda")
+    r64[:, 0, :] = s64.t()
+
+    def fp64():
+        emtrace._lib.call("em_trace_dev_f64", q64.data_ptr(), w.p, r64[0].data_ptr(),
+                          r64[1].data_ptr(), r64[2].data_ptr(), w.n, rows_n, w.n,
+                          emtrace.api.ctypes.byref(emtrace.make_step(rule_p)), None,
+                          emtrace.api._stream_handle())
+
+    _, k64 = timed(fp64, 2, 1)
+    i64 = float(w.p) * w.n * (rows_n - 1)
+    res["fp64_kernel"] = {"interactions_per_s": i64 / min(k64), "kernel_ms": min(k64) * 1e3,
+                          "rows": rows_n, "step_rule": "plain",
+                          "frac_fp64_peak": i64 / min(k64) * FLOP_PER_INTERACTION / 78.6e12,
+                          "note": "one lane per line, charges summed in index order; correctly "
+                                  "rounded sqrt and 3 divisions per interaction"}
+    del r64, q64, s64
+    # 6. the reference application's default problem (ElectroMag.cpp:43: 1,024
+    #    charges, 128 x 128 grid seeds, 2,500 rows, curvature step, resolution 1)
+    wl = emtrace.legacy.preset("default")
+    cl, sl = torch.from_numpy(wl.charges).cuda(), torch.from_numpy(wl.seeds).cuda()
+    rl = emtrace.seed_rows(sl, wl.len)
+    _, kl = timed(lambda: emtrace.trace_into(cl, rl, wl.n, rules.ref(1.0)), 3, 1)
+    res["legacy_default_preset"] = {
+        "interactions_per_s": float(wl.interactions) / min(kl), "kernel_ms": min(kl) * 1e3,
+        "charges": wl.p, "lines": wl.n, "rows": wl.len}
     return res
 
 

@@ -1,0 +1,388 @@
+// trace64.hip -- FP64 field-line tracer (the CalcField_CPU<double> counterpart).
+//
+// The reference's double path (ElectroMag/src/CPU_Implement.cpp:366-498 SSE2
+// curvature form, 38-102 generic plain form, dispatch 515-527) evaluated with
+// IEEE binary64 operations in the reference's source order: for each line the
+// charges are summed in index order 0..p-1, each PartField term is
+//   curvature / angle:  r*k*q / (|r|^2 sqrt(|r|^2))        (Electrostatics.h:38-51)
+//   plain:              r * ((k_e*q) / (|r|^2 sqrt(|r|^2)))   (Electrostatics.h:53-60)
+// with |r|^2 = (rx*rx + ry*ry) + rz*rz, and the step updates are those of the
+// FP32 kernels (em_device.hpp step_update) in double.  sqrt and division are
+// correctly rounded and the file is compiled without FMA contraction, so every
+// row equals the FP64 restatement of the reference algorithm bit for bit.
+//
+// Layout: one lane per line (a lane owns its line's whole charge sum, which
+// fixes the summation order); 256 lines per workgroup; charges staged through
+// LDS as 32-byte records, all of them once when they fit a tile, else tile by
+// tile each step.  The charge loop is unrolled so the independent per-charge
+// terms (a square root and three divisions each) overlap while the additions
+// stay in order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "em_internal.hpp"
+
+namespace em {
+
+struct StepP64 {
+    int rule;
+    double res, h0, hmin, hmax, turn;
+    int stop;
+};
+
+namespace {
+
+constexpr int kBlock64 = 256;
+constexpr uint32_t kTile64 = 2048;  // 64 KiB of LDS: two workgroups per CU
+constexpr double kCoulomb64 = 8.987551787E9;  // include/Electrostatics.h:27
+
+template <int RULE>
+__device__ __forceinline__ void term64(const double4 c, double px, double py, double pz,
+                                       double& ex, double& ey, double& ez)
+{
+    const double rx = px - c.x, ry = py - c.y, rz = pz - c.z;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double den = r2 * sqrt(r2);
+    if (RULE == kRulePlain) {
+        const double s = kCoulomb64 * c.w / den;
+        ex = rx * s;
+        ey = ry * s;
+        ez = rz * s;
+    } else {
+        const double k = RULE == kRuleRef ? kCoulomb64 : 1.0;
+        ex = rx * k * c.w / den;
+        ey = ry * k * c.w / den;
+        ez = rz * k * c.w / den;
+    }
+}
+
+template <int RULE>
+__device__ __forceinline__ void sum_range(const double4* __restrict__ tile, uint32_t count,
+                                          double px, double py, double pz, double& sx, double& sy,
+                                          double& sz)
+{
+    uint32_t i = 0;
+    for (; i + 4 <= count; i += 4) {
+        double e[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) term64<RULE>(tile[i + u], px, py, pz, e[u][0], e[u][1], e[u][2]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sx += e[u][0];
+            sy += e[u][1];
+            sz += e[u][2];
+        }
+    }
+    for (; i < count; ++i) {
+        double ex, ey, ez;
+        term64<RULE>(tile[i], px, py, pz, ex, ey, ez);
+        sx += ex;
+        sy += ey;
+        sz += ez;
+    }
+}
+
+struct Line64 {
+    double px, py, pz;  // current point
+    double Px, Py, Pz;  // seed (the reference's prevAccum)
+    double ux, uy, uz;  // previous direction (angle rule)
+    double h;           // previous step length (angle rule)
+    bool alive;
+    uint32_t steps;
+};
+
+template <int RULE>
+__device__ __forceinline__ void step64(const StepP64& sp, uint32_t s, double ex, double ey,
+                                       double ez, Line64& st)
+{
+    if (RULE == kRulePlain) {
+        const double len = sqrt(ex * ex + ey * ey + ez * ez);
+        const double sc = sp.res * len;
+        st.px = st.px + ex / sc;
+        st.py = st.py + ey / sc;
+        st.pz = st.pz + ez / sc;
+    } else if (RULE == kRuleRef) {
+        double k = ex * ex + ey * ey + ez * ez;
+        const double ix = ex - st.Px, iy = ey - st.Py, iz = ez - st.Pz;
+        const double cx = iy * st.Pz - iz * st.Py;
+        const double cy = iz * st.Px - ix * st.Pz;
+        const double cz = ix * st.Py - iy * st.Px;
+        k = sqrt(cx * cx + cy * cy + cz * cz) / (k * sqrt(k));
+        double sc = (k + 1.0) * sp.res;
+        sc = sc * sqrt(ex * ex + ey * ey + ez * ez);
+        st.px = st.px + ex / sc;
+        st.py = st.py + ey / sc;
+        st.pz = st.pz + ez / sc;
+    } else {
+        const double len = sqrt(ex * ex + ey * ey + ez * ez);
+        const double ux = ex / len, uy = ey / len, uz = ez / len;
+        if (sp.stop && s > 1 && ux * st.ux + uy * st.uy + uz * st.uz < 0.0) {
+            st.alive = false;  // went through a sink: the line ends at x_{s-1}
+            return;
+        }
+        double h = sp.h0;
+        if (s > 1) {
+            const double dx = ux - st.ux, dy = uy - st.uy, dz = uz - st.uz;
+            const double c = sqrt(dx * dx + dy * dy + dz * dz);
+            const double th = sp.turn * st.h;
+            h = (c * sp.hmax > th) ? th / c : sp.hmax;
+            h = h > sp.hmin ? h : sp.hmin;
+        }
+        st.px = st.px + h * ux;
+        st.py = st.py + h * uy;
+        st.pz = st.pz + h * uz;
+        st.ux = ux;
+        st.uy = uy;
+        st.uz = uz;
+        st.h = h;
+    }
+    st.steps = s;
+}
+
+template <int RULE>
+__global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict__ charges,
+                                                      uint32_t p, double* __restrict__ x,
+                                                      double* __restrict__ y,
+                                                      double* __restrict__ z, uint32_t n,
+                                                      uint32_t len, uint64_t pitch, StepP64 sp,
+                                                      uint32_t tile, uint32_t* steps_out)
+{
+    extern __shared__ double4 lds64[];
+    const uint32_t line = blockIdx.x * kBlock64 + threadIdx.x;
+    const bool valid = line < n;
+    Line64 st{};
+    if (valid) {
+        st.px = st.Px = x[line];
+        st.py = st.Py = y[line];
+        st.pz = st.Pz = z[line];
+    }
+    st.h = sp.h0;
+    st.alive = valid;
+    const bool single = p <= tile;
+    if (single) {
+        for (uint32_t i = threadIdx.x; i < p; i += kBlock64) lds64[i] = charges[i];
+        __syncthreads();
+    }
+    for (uint32_t s = 1; s < len; ++s) {
+        // the workgroup skips the field once none of its lines is alive
+        if (__syncthreads_or(st.alive)) {
+            double ex = 0.0, ey = 0.0, ez = 0.0;
+            if (single) {
+                if (st.alive) sum_range<RULE>(lds64, p, st.px, st.py, st.pz, ex, ey, ez);
+            } else {
+                for (uint32_t t0 = 0; t0 < p; t0 += tile) {
+                    const uint32_t cnt = min(tile, p - t0);
+                    __syncthreads();
+                    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock64) lds64[i] = charges[t0 + i];
+                    __syncthreads();
+                    if (st.alive) sum_range<RULE>(lds64, cnt, st.px, st.py, st.pz, ex, ey, ez);
+                }
+            }
+            if (st.alive) step64<RULE>(sp, s, ex, ey, ez, st);
+        }
+        if (valid) {
+            const uint64_t at = (uint64_t)s * pitch + line;
+            x[at] = st.px;
+            y[at] = st.py;
+            z[at] = st.pz;
+        }
+    }
+    if (valid && steps_out) steps_out[line] = st.steps;
+}
+
+int validate64(size_t p, size_t n, size_t len, const StepP64& sp)
+{
+    // the reference's order: n, resolution, rows (CPU_Implement.cpp:43-52, 213-230)
+    if (!n) EM_FAIL(EM_ERR_NO_LINES, "n == 0");
+    if (sp.rule == kRulePlain || sp.rule == kRuleRef) {
+        if (sp.res == 0.0) EM_FAIL(EM_ERR_RESOLUTION, "resolution == 0");
+        if (!std::isfinite(sp.res)) EM_FAIL(EM_ERR_INVALID, "resolution not finite");
+    } else if (sp.rule == kRuleAngle) {
+        if (!(sp.h0 > 0.0 && sp.hmin > 0.0 && sp.hmax >= sp.hmin && sp.turn > 0.0) ||
+            !std::isfinite(sp.hmax) || !std::isfinite(sp.h0))
+            EM_FAIL(EM_ERR_INVALID, "angle rule needs h0 > 0, 0 < hmin <= hmax, turn > 0");
+    } else {
+        EM_FAIL(EM_ERR_INVALID, "unknown step rule %d", sp.rule);
+    }
+    if (sp.stop && sp.rule != kRuleAngle)
+        EM_FAIL(EM_ERR_INVALID, "stop is defined for the angle rule only");
+    if (len < 2) EM_FAIL(EM_ERR_LENGTH, "len %zu < 2", len);
+    if (p > 0xFFFFFFFFull || n > 0xFFFFFFFFull || len > 0xFFFFFFFFull)
+        EM_FAIL(EM_ERR_INVALID, "p, n and len must each fit 32 bits");
+    return EM_OK;
+}
+
+StepP64 from_step(const em_step& st)
+{
+    StepP64 sp;
+    sp.rule = st.rule;
+    sp.res = st.resolution;
+    sp.h0 = st.h0;
+    sp.hmin = st.hmin;
+    sp.hmax = st.hmax;
+    sp.turn = st.turn;
+    sp.stop = st.stop ? 1 : 0;
+    return sp;
+}
+
+int launch64(const double* charges, uint32_t p, double* x, double* y, double* z, uint32_t n,
+             uint32_t len, uint64_t pitch, const StepP64& sp, uint32_t* steps, hipStream_t stream)
+{
+    const uint32_t tile = p <= kTile64 ? std::max<uint32_t>(p, 1) : kTile64 / 2;
+    const size_t lds = (size_t)tile * sizeof(double4);
+    const dim3 grid((n + kBlock64 - 1) / kBlock64), block(kBlock64);
+    const double4* q = reinterpret_cast<const double4*>(charges);
+    switch (sp.rule) {
+    case kRulePlain:
+        hipLaunchKernelGGL(k_trace64<kRulePlain>, grid, block, lds, stream, q, p, x, y, z, n, len,
+                           pitch, sp, tile, steps);
+        break;
+    case kRuleRef:
+        hipLaunchKernelGGL(k_trace64<kRuleRef>, grid, block, lds, stream, q, p, x, y, z, n, len,
+                           pitch, sp, tile, steps);
+        break;
+    default:
+        hipLaunchKernelGGL(k_trace64<kRuleAngle>, grid, block, lds, stream, q, p, x, y, z, n, len,
+                           pitch, sp, tile, steps);
+        break;
+    }
+    EM_HIP(hipGetLastError());
+    return EM_OK;
+}
+
+double now64()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Host arrays in, host arrays out; device buffers live for the call only.
+int host_trace64(const double* charges, size_t p, double* x, double* y, double* z, size_t n,
+                 size_t len, const StepP64& sp, em_perf* perf, uint32_t* steps)
+{
+    const double t0 = now64();
+    if (perf) {
+        perf->progress = 0.0;
+        perf->time_s = perf->kernel_s = perf->h2d_s = perf->d2h_s = 0.0;
+        perf->h2d_bytes = perf->d2h_bytes = perf->gflops = 0.0;
+    }
+    EM_TRY(validate64(p, n, len, sp));
+    if ((p && !charges) || !x || !y || !z) EM_FAIL(EM_ERR_INVALID, "NULL array");
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+
+    struct Res {  // released on every return path
+        void* q = nullptr;
+        void* lines = nullptr;
+        void* st = nullptr;
+        hipStream_t stream = nullptr;
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Res()
+        {
+            if (stream) (void)hipStreamSynchronize(stream);
+            if (q) (void)hipFree(q);
+            if (lines) (void)hipFree(lines);
+            if (st) (void)hipFree(st);
+            for (auto& e : ev)
+                if (e) (void)hipEventDestroy(e);
+            if (stream) (void)hipStreamDestroy(stream);
+        }
+    } r;
+    const size_t row = n * sizeof(double);
+    EM_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+    for (auto& e : r.ev) EM_HIP(hipEventCreate(&e));
+    EM_HIP(hipMalloc(&r.q, std::max<size_t>(p, 1) * sizeof(double4)));
+    EM_HIP(hipMalloc(&r.lines, 3 * len * row));
+    if (steps) EM_HIP(hipMalloc(&r.st, n * sizeof(uint32_t)));
+    double* dx = static_cast<double*>(r.lines);
+    double* dy = dx + len * n;
+    double* dz = dy + len * n;
+
+    EM_HIP(hipEventRecord(r.ev[0], r.stream));
+    if (p) EM_HIP(hipMemcpyAsync(r.q, charges, p * sizeof(double4), hipMemcpyHostToDevice, r.stream));
+    EM_HIP(hipMemcpyAsync(dx, x, row, hipMemcpyHostToDevice, r.stream));
+    EM_HIP(hipMemcpyAsync(dy, y, row, hipMemcpyHostToDevice, r.stream));
+    EM_HIP(hipMemcpyAsync(dz, z, row, hipMemcpyHostToDevice, r.stream));
+    EM_HIP(hipEventRecord(r.ev[1], r.stream));
+    EM_TRY(launch64(static_cast<const double*>(r.q), (uint32_t)p, dx, dy, dz, (uint32_t)n,
+                    (uint32_t)len, n, sp, static_cast<uint32_t*>(r.st), r.stream));
+    EM_HIP(hipEventRecord(r.ev[2], r.stream));
+    const size_t out_bytes = (len - 1) * row;
+    EM_HIP(hipMemcpyAsync(x + n, dx + n, out_bytes, hipMemcpyDeviceToHost, r.stream));
+    EM_HIP(hipMemcpyAsync(y + n, dy + n, out_bytes, hipMemcpyDeviceToHost, r.stream));
+    EM_HIP(hipMemcpyAsync(z + n, dz + n, out_bytes, hipMemcpyDeviceToHost, r.stream));
+    if (steps)
+        EM_HIP(hipMemcpyAsync(steps, r.st, n * sizeof(uint32_t), hipMemcpyDeviceToHost, r.stream));
+    EM_HIP(hipEventRecord(r.ev[3], r.stream));
+    EM_HIP(hipEventSynchronize(r.ev[3]));
+
+    float ms[3] = {0, 0, 0};
+    EM_HIP(hipEventElapsedTime(&ms[0], r.ev[0], r.ev[1]));
+    EM_HIP(hipEventElapsedTime(&ms[1], r.ev[1], r.ev[2]));
+    EM_HIP(hipEventElapsedTime(&ms[2], r.ev[2], r.ev[3]));
+    if (perf) {
+        const double ks = ms[1] * 1e-3, dn = (double)n, dl = (double)len, dp = (double)p;
+        perf->time_s = now64() - t0;
+        perf->kernel_s = ks;
+        perf->h2d_s = ms[0] * 1e-3;
+        perf->d2h_s = ms[2] * 1e-3;
+        perf->h2d_bytes = (double)(p * sizeof(double4) + 3 * row);
+        perf->d2h_bytes = (double)(3 * out_bytes);
+        perf->interactions = dp * dn * (dl - 1.0);
+        perf->interactions_per_s = ks > 0 ? perf->interactions / ks : 0.0;
+        perf->line_steps_per_s = ks > 0 ? dn * (dl - 1.0) / ks : 0.0;
+        // CPU_Implement.cpp:347-349
+        perf->gflops = (dn * (dl - 1.0) * (dp * 18.0 + 13.0)) / perf->time_s / 1e9;
+        perf->lanes_per_line = 1;
+        perf->tile = (int32_t)(p <= kTile64 ? std::max<size_t>(p, 1) : kTile64 / 2);
+        perf->block = kBlock64;
+        perf->progress = 1.0;
+    }
+    return EM_OK;
+}
+
+}  // namespace
+}  // namespace em
+
+using namespace em;
+
+extern "C" {
+
+int em_calc_lines_f64(const double* charges, size_t p, double* x, double* y, double* z, size_t n,
+                      size_t len, double resolution, int use_curvature, em_perf* perf)
+{
+    clear_error();
+    StepP64 sp{};
+    sp.rule = use_curvature ? kRuleRef : kRulePlain;
+    sp.res = resolution;
+    return host_trace64(charges, p, x, y, z, n, len, sp, perf, nullptr);
+}
+
+int em_trace_host_f64(const double* charges, size_t p, double* x, double* y, double* z, size_t n,
+                      size_t len, const em_step* step, em_perf* perf, uint32_t* steps)
+{
+    clear_error();
+    if (!step) EM_FAIL(EM_ERR_INVALID, "step is NULL");
+    return host_trace64(charges, p, x, y, z, n, len, from_step(*step), perf, steps);
+}
+
+int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x, double* d_y, double* d_z,
+                     size_t n, size_t len, size_t pitch, const em_step* step, uint32_t* d_steps,
+                     void* stream)
+{
+    clear_error();
+    if (!step) EM_FAIL(EM_ERR_INVALID, "step is NULL");
+    const StepP64 sp = from_step(*step);
+    EM_TRY(validate64(p, n, len, sp));
+    if ((p && !d_charges) || !d_x || !d_y || !d_z) EM_FAIL(EM_ERR_INVALID, "NULL device pointer");
+    if (pitch < n) EM_FAIL(EM_ERR_INVALID, "pitch %zu < n %zu", pitch, n);
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+    return launch64(d_charges, (uint32_t)p, d_x, d_y, d_z, (uint32_t)n, (uint32_t)len, pitch, sp,
+                    d_steps, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -96,6 +96,10 @@ SIGNATURES = {
     "em_batch_ckpt_floats": (SZ, [P, U32]),
     "em_batch_run_ckpt": (I, [P, P, P, P, P, P, U32, P, P]),
     "em_batch_adjoint": (I, [P, P, P, P, P, P, U32, P, P, P, P, P]),
+    "em_calc_lines_f64": (I, [P, SZ, P, P, P, SZ, SZ, ctypes.c_double, I, ctypes.POINTER(em_perf)]),
+    "em_trace_host_f64": (I, [P, SZ, P, P, P, SZ, SZ, ctypes.POINTER(em_step),
+                              ctypes.POINTER(em_perf), P]),
+    "em_trace_dev_f64": (I, [P, SZ, P, P, P, SZ, SZ, SZ, ctypes.POINTER(em_step), P, P]),
     "em_legacy_rand": (I, [U32, P, SZ]),
     "em_legacy_problem_f32": (I, [P, SZ, U32, P, P, P, SZ, SZ, SZ, I, U32]),
 }

@@ -80,6 +80,37 @@ def trace_host(charges: np.ndarray, seeds: np.ndarray, length: int, rule: StepRu
     return rows, perf.as_dict(), steps
 
 
+def calc_lines64(charges: np.ndarray, x: np.ndarray, y: np.ndarray, z: np.ndarray, n: int,
+                 resolution: float, use_curvature: bool = False):
+    """Drop-in for ``CalcField_CPU<double>`` (IEEE binary64 kernel); as calc_lines."""
+    charges = np.ascontiguousarray(charges, dtype=np.float64)
+    for a in (x, y, z):
+        if a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise TypeError("x, y, z must be contiguous float64 arrays (written in place)")
+    length = x.size // n if n else 0
+    perf = em_perf()
+    rc = lib().em_calc_lines_f64(_np_ptr(charges), charges.shape[0], _np_ptr(x), _np_ptr(y),
+                                 _np_ptr(z), n, length, float(resolution), int(bool(use_curvature)),
+                                 ctypes.byref(perf))
+    return rc, perf.as_dict()
+
+
+def trace_host64(charges: np.ndarray, seeds: np.ndarray, length: int, rule: StepRule):
+    """FP64 trace from host arrays.  Returns (rows[3, len, n] float64, perf, steps[n])."""
+    charges = np.ascontiguousarray(charges, dtype=np.float64)
+    seeds = np.asarray(seeds, dtype=np.float64)
+    n = seeds.shape[0]
+    rows = np.zeros((3, length, n), dtype=np.float64)
+    rows[:, 0, :] = seeds.T
+    steps = np.zeros(n, dtype=np.uint32)
+    perf = em_perf()
+    st = make_step(rule)
+    call("em_trace_host_f64", _np_ptr(charges), charges.shape[0], _np_ptr(rows[0]),
+         _np_ptr(rows[1]), _np_ptr(rows[2]), n, length, ctypes.byref(st), ctypes.byref(perf),
+         _np_ptr(steps))
+    return rows, perf.as_dict(), steps
+
+
 # ---------------------------------------------------------------------------
 # device tensors
 def _torch():
@@ -157,6 +188,24 @@ def trace(charges, seeds, length: int, rule: StepRule, launch=None, with_steps: 
     steps = torch.empty(seeds.shape[0], dtype=torch.int32, device=seeds.device)
     trace_into(charges, rows, seeds.shape[0], rule, launch, steps=steps)
     return rows, steps
+
+
+def trace64(charges, seeds, length: int, rule: StepRule, with_steps: bool = False):
+    """FP64 trace from device seeds (n, 3) float64; charges (p, 4) float64.
+    Returns float64 rows (3, len, n) [and int32 step counts]."""
+    torch = _torch()
+    _dev_check(charges, "charges", torch.float64)
+    if charges.dim() != 2 or charges.shape[1] != 4:
+        raise ValueError("charges must be (p, 4)")
+    n = seeds.shape[0]
+    rows = torch.empty((3, length, n), dtype=torch.float64, device=seeds.device)
+    rows[:, 0, :] = seeds.t()
+    steps = torch.empty(n, dtype=torch.int32, device=seeds.device) if with_steps else None
+    st = make_step(rule)
+    call("em_trace_dev_f64", charges.data_ptr(), charges.shape[0], rows[0].data_ptr(),
+         rows[1].data_ptr(), rows[2].data_ptr(), n, length, n, ctypes.byref(st),
+         steps.data_ptr() if steps is not None else None, _stream_handle())
+    return (rows, steps) if with_steps else rows
 
 
 def field(charges, points, scale: float = 1.0, launch=None):

@@ -105,6 +105,27 @@ int em_trace_host_f32(const float* charges, size_t p, float* x, float* y,
                       float* z, size_t n, size_t len, const em_step* step,
                       const em_launch* launch, em_perf* perf, uint32_t* steps);
 
+/* ---- FP64 ------------------------------------------------------------------
+ * Replaces: template<> int CalcField_CPU<double>(...) (CPU Implement.h:25-30,
+ *   CPU_Implement.cpp:366-498, 515-527).  charges: p records {x, y, z, q} of
+ * float64 (pointCharge<double>); lines: float64 step-major as above.  Every
+ * operation is IEEE binary64 in the reference's source order (charges summed
+ * in index order, correctly rounded sqrt and division, no fused
+ * multiply-add), so the rows equal the FP64 restatement of the reference
+ * algorithm bit for bit.  Same status codes as the FP32 forms. */
+int em_calc_lines_f64(const double* charges, size_t p, double* x, double* y,
+                      double* z, size_t n, size_t len, double resolution,
+                      int use_curvature, em_perf* perf);
+/* Any step rule (parameters are the em_step floats, widened exactly). */
+int em_trace_host_f64(const double* charges, size_t p, double* x, double* y,
+                      double* z, size_t n, size_t len, const em_step* step,
+                      em_perf* perf, uint32_t* steps);
+/* Device pointers, asynchronous on `stream`; pitch >= n. */
+int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x,
+                     double* d_y, double* d_z, size_t n, size_t len,
+                     size_t pitch, const em_step* step, uint32_t* d_steps,
+                     void* stream);
+
 /* ---- device-pointer forms (asynchronous on `stream`, a hipStream_t) -------
  * Replaces the OpenCL functor's device path (GPGPU_Segment/src/
  * CL_Electrostatics.cpp:396-513 MainFunctor; Electrostatics.cl.c:282-367).

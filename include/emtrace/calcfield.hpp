@@ -1,7 +1,7 @@
 // emtrace/calcfield.hpp -- C++ drop-in for the reference's calc-lines call.
 //
 // Replaces   CalcField_CPU<float>(fieldLines, pointCharges, n, resolution,
-//                                 perfData, useCurvature)
+//                                 perfData, useCurvature)   (and <double>)
 //   (ElectroMag/src/CPU Implement.h:25-30, called at ElectroMag.cpp:305-306)
 // and the GPU stub CalcField(...) (GPGPU_Segment/src/Electrostatics.cpp:28-42)
 // with emtrace::CalcField_GPU, which has the same arguments and return codes
@@ -40,12 +40,13 @@ inline int calc_lines(float* x, float* y, float* z, const void* q, size_t p, siz
     return em_calc_lines_f32(static_cast<const float*>(q), p, x, y, z, n, len, (float)resolution,
                              curvature ? 1 : 0, perf);
 }
-// FP64 lines have no GPU kernel yet (the reference's GPU stub for doubles is
-// not implemented either, GPGPU_Segment/src/Electrostatics.cpp:36-42).
-inline int calc_lines(double*, double*, double*, const void*, size_t, size_t, size_t, double, bool,
-                      em_perf*)
+// FP64 lines: the IEEE double kernel (the reference's GPU stub for doubles is
+// not implemented, GPGPU_Segment/src/Electrostatics.cpp:36-42).
+inline int calc_lines(double* x, double* y, double* z, const void* q, size_t p, size_t n,
+                      size_t len, double resolution, bool curvature, em_perf* perf)
 {
-    return EM_ERR_UNSUPPORTED;
+    return em_calc_lines_f64(static_cast<const double*>(q), p, x, y, z, n, len, resolution,
+                             curvature ? 1 : 0, perf);
 }
 }  // namespace detail
 

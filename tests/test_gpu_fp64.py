@@ -1,0 +1,163 @@
+"""FP64 kernel (the CalcField_CPU<double> counterpart).
+
+* every row equals the oracle's FP64 restatement bit for bit -- plain, ref,
+  angle with and without line ends, single- and multi-tile charge staging,
+  partial workgroups, pitch > n on the device form;
+* against the reference's own double rows (golden vectors) it is as close as
+  the restatement is (-ffast-math reassociation in the reference build);
+* the drop-in calc_lines64 keeps the reference's status codes;
+* at the full config-0 size (1,000 charges, 8,192 lines, 2,500 rows) the FP32
+  kernel drifts from the FP64 kernel no more than the CPU FP32 path does;
+* the command line with --cpuprecision=double --gpuprecision=double writes
+  identical CPU and GPU text.
+"""
+import dataclasses
+import glob
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from emtrace import gen, legacy, rules
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _as_f32_params(rule):
+    """The em_step carries float32 parameters: give the oracle the same values."""
+    f = lambda v: float(np.float32(v))  # noqa: E731
+    return dataclasses.replace(rule, resolution=f(rule.resolution), h0=f(rule.h0), hmin=f(rule.hmin),
+                               hmax=f(rule.hmax), turn=f(rule.turn))
+
+
+CASES = [
+    # (name, p, n, half_side, len, rule)
+    ("plain", 300, 512, 10.0, 200, rules.plain(10.0)),
+    ("ref", 300, 512, 10.0, 200, rules.ref(10.0)),
+    ("angle_stop", 300, 448, 5.0, 300, rules.angle()),
+    ("angle_nostop", 300, 448, 5.0, 300, rules.angle(stop=False)),
+    ("ragged_block", 7, 130, 3.0, 50, rules.angle()),
+    ("multi_tile_plain", 5000, 128, 20.0, 30, rules.plain(10.0)),
+    ("multi_tile_angle", 4100, 192, 20.0, 40, rules.angle()),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp64_bitwise_equals_restatement(gpu, orc, case):
+    name, p, n, half, length, rule = case
+    rule = _as_f32_params(rule)
+    w = gen.plain_config(p, n, half, length, seed=11)
+    ch = w.charges.astype(np.float64)
+    sd = w.seeds.astype(np.float64)
+    rc, want, want_steps = orc.calc_lines(ch, sd, length, rule, dtype=np.float64, with_steps=True)
+    assert rc == 0
+    rows, steps = gpu.trace64(torch.from_numpy(ch).cuda(), torch.from_numpy(sd).cuda(), length, rule,
+                              with_steps=True)
+    rows = rows.cpu().numpy()
+    same = (rows == want) | (np.isnan(rows) & np.isnan(want))
+    print(f"[fp64 {name}] identical {same.mean():.6f}; steps identical "
+          f"{np.mean(steps.cpu().numpy() == want_steps.astype(np.int64)):.4f}", flush=True)
+    assert same.all()
+    assert np.array_equal(steps.cpu().numpy(), want_steps.astype(np.int32))
+
+
+def test_fp64_host_and_pitched_device_forms(gpu, orc):
+    rule = _as_f32_params(rules.angle())
+    w = gen.plain_config(200, 300, 5.0, 80, seed=4)
+    ch, sd = w.charges.astype(np.float64), w.seeds.astype(np.float64)
+    rc, want = orc.calc_lines(ch, sd, w.len, rule, dtype=np.float64)
+    rows, perf, steps = gpu.trace_host64(ch, sd, w.len, rule)
+    assert np.array_equal(rows, want)
+    assert perf["progress"] == 1.0 and perf["kernel_s"] > 0 and perf["interactions"] == 200 * 300 * 79
+    # device form with pitch > n: the padding columns are left alone
+    import ctypes
+    from emtrace import api
+    from emtrace._lib import call
+    pitch = 320
+    buf = torch.full((3, w.len, pitch), 7.0, dtype=torch.float64, device="cuda")
+    buf[:, 0, :w.n] = torch.from_numpy(sd).cuda().t()
+    st = api.make_step(rule)
+    q = torch.from_numpy(ch).cuda()
+    call("em_trace_dev_f64", q.data_ptr(), w.p, buf[0].data_ptr(), buf[1].data_ptr(),
+         buf[2].data_ptr(), w.n, w.len, pitch, ctypes.byref(st), None, api._stream_handle())
+    torch.cuda.synchronize()
+    got = buf.cpu().numpy()
+    assert np.array_equal(got[:, :, :w.n], want)
+    assert (got[:, 1:, w.n:] == 7.0).all()
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "ref_f64_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[:-4])
+def test_fp64_vs_reference_double_golden(gpu, orc, path):
+    g = np.load(path)
+    res = float(g["resolution"])
+    rule = rules.ref(res) if int(g["use_curvature"]) else rules.plain(res)
+    length = g["rows"].shape[1]
+    x = np.ascontiguousarray(np.repeat(g["seeds"][:, 0][None], length, 0))
+    y = np.ascontiguousarray(np.repeat(g["seeds"][:, 1][None], length, 0))
+    z = np.ascontiguousarray(np.repeat(g["seeds"][:, 2][None], length, 0))
+    n = g["seeds"].shape[0]
+    rc, perf = gpu.calc_lines64(g["charges"], x, y, z, n, res, bool(int(g["use_curvature"])))
+    assert rc == 0 and perf["progress"] == 1.0
+    got = np.stack([x, y, z])
+    rc, want = orc.calc_lines(g["charges"], g["seeds"], length, rule, dtype=np.float64)
+    assert np.array_equal(got, want)
+    assert np.abs(got - g["rows"]).max() < 1e-9
+
+
+def test_fp64_dropin_status_codes(gpu):
+    ch = np.zeros((4, 4))
+    x = np.zeros(64 * 10)
+    y, z = x.copy(), x.copy()
+    assert gpu.calc_lines64(ch, x, y, z, 0, 1.0, True)[0] == 1
+    assert gpu.calc_lines64(ch, x, y, z, 64, 0.0, True)[0] == 2
+    x1, y1, z1 = np.zeros(64), np.zeros(64), np.zeros(64)
+    assert gpu.calc_lines64(ch, x1, y1, z1, 64, 1.0, True)[0] == 3
+
+
+def test_fp32_kernel_vs_fp64_kernel_full_config0(gpu, orc):
+    """Full config 0 on both kernels: the FP64 kernel stands in for the oracle at
+    a size the CPU restatement would take minutes on.  FP32 rows drift from FP64
+    over 2,500 steps on any hardware; the GPU's FP32 rows must drift no more
+    than the CPU FP32 restatement's (sampled lines, prefixes before the first
+    approach within 0.05 of a charge)."""
+    from helpers import line_errors, prefix_lengths
+    w = gen.config0()
+    rule = rules.ref(10.0)
+    q32 = torch.from_numpy(w.charges).cuda()
+    r32 = gpu.trace(q32, torch.from_numpy(w.seeds).cuda(), w.len, rule).cpu().numpy()
+    r64 = gpu.trace64(q32.double(), torch.from_numpy(w.seeds).cuda().double(), w.len,
+                      _as_f32_params(rule)).cpu().numpy()
+    assert np.isfinite(r64).all()
+    idx = np.arange(0, w.n, 16)
+    rc, c32 = orc.calc_lines(w.charges, w.seeds[idx], w.len, rule)
+    assert rc == 0
+    ref64 = r64[:, :, idx]
+    first, _ = prefix_lengths(ref64.astype(np.float32), w.charges)
+    e_gpu = line_errors(r32[:, :, idx], ref64, first)
+    e_cpu = line_errors(c32, ref64, first)
+    q = lambda e, f: float(np.quantile(e, f))  # noqa: E731
+    print(f"[config0 vs FP64 kernel, {idx.size} lines] gpu fp32 median {q(e_gpu, .5):.3e} p90 "
+          f"{q(e_gpu, .9):.3e} max {e_gpu.max():.3e}; cpu fp32 median {q(e_cpu, .5):.3e} p90 "
+          f"{q(e_cpu, .9):.3e} max {e_cpu.max():.3e}", flush=True)
+    assert q(e_gpu, 0.5) <= 2 * q(e_cpu, 0.5) + 1e-6
+    assert q(e_gpu, 0.9) <= 2 * q(e_cpu, 0.9) + 1e-6
+
+
+def test_cli_double_precision_cpu_equals_gpu(gpu, tmp_path):
+    import subprocess
+    cli = os.path.join(os.path.dirname(GOLDEN), "..", "electromag-with-cuda_amd", "bin", "electromag")
+    r = subprocess.run([cli, "--bogo", "--cpu", "--cpuprecision=double", "--gpuprecision=double",
+                        "--save", "--autoregress", "--nodisp"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = open(tmp_path / "data.txt").read().splitlines()
+    cpu = [s[6:] for s in lines if s.startswith(" CPUL ")]
+    gpu_ = [s[6:] for s in lines if s.startswith(" GPUL ")]
+    assert len(cpu) == 256 * 500 and cpu == gpu_
+    assert open(tmp_path / "regression.txt").read() == ""
+    w = legacy.preset("bogo")
+    assert lines[0].startswith(" CPUL [0][0] x: %g" % float(w.seeds[0, 0]))
