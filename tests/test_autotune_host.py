@@ -1,0 +1,39 @@
+"""Host logic of the launch-shape tuner (no GPU): candidate generation, the
+tie-break, and the shipped MI355X table."""
+import math
+
+from emtrace import autotune
+
+
+def test_candidates_fit_lds_and_granules():
+    for p, n, batch in ((16384, 65536, False), (1000, 8192, False), (8188, 713262, True),
+                        (7, 130, False)):
+        cands = autotune.candidates(p, n, 256, batch=batch)
+        assert cands
+        for S, block, tile in cands:
+            assert S in (1, 2, 4, 8, 16, 32) and block in (256, 512)
+            assert tile % max(256, S * autotune.GROUP) == 0
+            assert tile * 16 <= autotune.LDS_BYTES
+            if batch:
+                assert S <= 8
+
+
+def test_pick_prefers_fewer_lanes_among_ties():
+    rep = [(1, 256, 512, 10.5), (2, 256, 512, 10.04), (4, 512, 512, 10.0), (8, 512, 512, 9.99)]
+    assert autotune._pick(rep)[:3] == (2, 256, 512)
+    rep = [(1, 256, 512, 12.0), (4, 512, 512, 10.0), (8, 512, 512, float("inf"))]
+    assert autotune._pick(rep)[:3] == (4, 512, 512)
+
+
+def test_shipped_shapes_are_candidates_of_their_problem():
+    for key, (S, block, tile, _) in autotune.SHIPPED.items():
+        kind, arch, cus, *rest = key.split(":")
+        assert arch == "gfx950" and cus == "256"
+        if kind == "plain":
+            p, n = int(rest[0]), int(rest[1])
+            cands = autotune.candidates(p, n, 256)
+        else:
+            p, n = int(rest[1]), int(rest[2])
+            cands = autotune.candidates(p, n, 256, batch=True)
+            cands += [(s, b, min(t, int(math.ceil(p / 256) * 256))) for s, b, t in cands]
+        assert (S, block, tile) in cands, key
