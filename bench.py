@@ -189,6 +189,14 @@ def main():
                      "kernel_ms": k_avg * 1e3, "flop_per_launch": inter * FLOP_PER_INTERACTION,
                      "algorithmic_bytes_per_launch": 12.0 * w.n * (w.len - 1)},
     }
+    hw_flop = profiled_value("trace_config1_valu_counters.md", "fp32_flop_per_launch",
+                             args.workload, shape)
+    if hw_flop is not None:
+        out["roofline"]["rocprof_fp32"] = {
+            "flop_per_launch": hw_flop, "achieved": hw_flop / k_avg / 1e12,
+            "frac": hw_flop / k_avg / 1e12 / PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "source": "profiles/trace_config1_valu_counters.md (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 "
+                      "x 64 per dispatch) over this run's kernel time"}
     traffic = profiled_traffic(args.workload, shape)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
@@ -200,16 +208,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def profiled_traffic(workload, shape):
-    """HBM bytes per launch from the committed counter profile, when this run
-    used the launch shape that was profiled (lanes 4, tile 4096, block 512)."""
-    path = os.path.join(ROOT, "profiles", "trace_config1_hbm_traffic.md")
+def profiled_value(name, key, workload, shape):
+    """A per-launch figure from a committed counter profile, when this run used
+    the launch shape that was profiled (lanes 4, tile 4096, block 512)."""
+    path = os.path.join(ROOT, "profiles", name)
     if workload != "config1" or shape != (4, 4096, 512) or not os.path.exists(path):
         return None
     for line in open(path):
-        if "traffic_bytes_per_launch:" in line:
-            return float(line.split(":")[1].strip(" *\n"))
+        if f"{key}:" in line:
+            return float(line.split(f"{key}:")[1].split("*")[0].strip(" *\n"))
     return None
+
+
+def profiled_traffic(workload, shape):
+    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE)."""
+    return profiled_value("trace_config1_hbm_traffic.md", "traffic_bytes_per_launch", workload,
+                          shape)
 
 
 def extras(w, charges, seeds, plain_value, launch):
