@@ -20,8 +20,8 @@ absent, on a bounded slice of the same workload.
 Extra fields (skipped with --no-extras): the same workload with line ends at
 sinks (useful interactions only), the batched workload (512 configurations,
 one hipGraph-replayed persistent launch), the differentiation workload
-(forward + adjoint time over forward time), the inverse fit, the FP64 kernel
-and the reference application's default problem.
+(forward + adjoint time over forward time), the inverse fit, the FP64 kernel,
+the reference application's default problem and the particle step.
 """
 from __future__ import annotations
 
@@ -144,7 +144,7 @@ def main():
         launch, tuned = None, "heuristic"
     else:
         launch, report = autotune.tune_plain(charges, seeds, rule)
-        tuned = "autotuned"
+        tuned = "autotuned (cache or shipped table)" if report[0][-1] == "cached" else "autotuned"
     rows = emtrace.seed_rows(seeds, w.len)
     steps_buf = torch.empty(w.n, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
@@ -337,6 +337,27 @@ def extras(w, charges, seeds, plain_value, launch):
     res["legacy_default_preset"] = {
         "interactions_per_s": float(wl.interactions) / min(kl), "kernel_ms": min(kl) * 1e3,
         "charges": wl.p, "lines": wl.n, "rows": wl.len}
+    del rl, cl, sl
+    # 7. particle step (Particle_System.cpp:108-190): 65,536 particles among the
+    #    16,384 config-1 charges, one step with the written-back state
+    import numpy as np
+    rng = np.random.default_rng(0)
+    npart = 65536
+    P = np.zeros((npart, 8), np.float32)
+    P[:, :3] = rng.uniform(-50, 50, (npart, 3))
+    P[:, 3] = rng.uniform(-1e-3, 1e-3, npart)
+    P[:, 4:7] = rng.normal(0, 1, (npart, 3))
+    P[:, 7] = 1.0
+    dP = torch.from_numpy(P).cuda()
+    dA = torch.zeros((npart, 3), dtype=torch.float32, device="cuda")
+    dO = torch.empty_like(dP)
+    _, kv = timed(lambda: emtrace.verlet_step_dev(dP, charges, dA, [-50] * 3, [50] * 3, 1e-3, out=dO),
+                  3, 1)
+    pairs = float(npart) * (w.p + npart - 1)
+    res["particle_step"] = {"particles": npart, "static_charges": w.p, "interactions": pairs,
+                            "interactions_per_s": pairs / min(kv), "kernel_ms": min(kv) * 1e3,
+                            "note": "static terms 12 VALU + rsq, particle-particle terms (Coulomb + "
+                                    "Biot-Savart, self masked) 22 VALU + rsq"}
     return res
 
 

@@ -8,13 +8,13 @@ from . import gen, legacy, rules
 from ._lib import EmError, LibraryMissing, lib, LIB_PATH
 from .api import (BatchLayout, BatchPlan, calc_lines, calc_lines64, ckpt_size, field, make_launch,
                   make_step, pack_batch, seed_rows, trace, trace64, trace_host, trace_host64,
-                  trace_into)
+                  trace_into, verlet_step, verlet_step_dev)
 from .rules import ANGLE, PLAIN, REF, StepRule
 from .autograd import adjoint, batch_trace, trace_lines
 
 __all__ = [
     "gen", "legacy", "rules", "EmError", "LibraryMissing", "lib", "LIB_PATH", "BatchLayout", "BatchPlan",
     "calc_lines", "calc_lines64", "ckpt_size", "field", "make_launch", "make_step", "pack_batch", "seed_rows",
-    "trace", "trace64", "trace_host", "trace_host64", "trace_into", "ANGLE", "PLAIN", "REF", "StepRule", "adjoint",
+    "trace", "trace64", "trace_host", "trace_host64", "trace_into", "verlet_step", "verlet_step_dev", "ANGLE", "PLAIN", "REF", "StepRule", "adjoint",
     "batch_trace", "trace_lines",
 ]

@@ -100,6 +100,8 @@ SIGNATURES = {
     "em_trace_host_f64": (I, [P, SZ, P, P, P, SZ, SZ, ctypes.POINTER(em_step),
                               ctypes.POINTER(em_perf), P]),
     "em_trace_dev_f64": (I, [P, SZ, P, P, P, SZ, SZ, SZ, ctypes.POINTER(em_step), P, P]),
+    "em_verlet_step_f32": (I, [P, SZ, P, SZ, P, P, P, F, I, ctypes.POINTER(em_perf)]),
+    "em_verlet_step_dev_f32": (I, [P, SZ, P, SZ, P, P, P, F, P, P]),
     "em_legacy_rand": (I, [U32, P, SZ]),
     "em_legacy_problem_f32": (I, [P, SZ, U32, P, P, P, SZ, SZ, SZ, I, U32]),
 }

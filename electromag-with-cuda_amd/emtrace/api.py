@@ -111,6 +111,27 @@ def trace_host64(charges: np.ndarray, seeds: np.ndarray, length: int, rule: Step
     return rows, perf.as_dict(), steps
 
 
+def verlet_step(particles: np.ndarray, charges: np.ndarray, accel: np.ndarray, box_min, box_max,
+                dt: float, write_back: bool = False):
+    """The particle step (emgVerletStep, Particle_System.cpp:108-190) on the GPU
+    from host arrays.  particles (n, 8) float32 {x, y, z, q, vx, vy, vz, m};
+    charges (p, 4); accel (n, 3) = a(t), overwritten with a(t + dt); with
+    write_back the particles are advanced in place.  Returns perf."""
+    for a, name, w in ((particles, "particles", 8), (accel, "accel", 3)):
+        if a.dtype != np.float32 or not a.flags.c_contiguous or a.ndim != 2 or a.shape[1] != w:
+            raise TypeError(f"{name} must be a contiguous float32 (n, {w}) array (written in place)")
+    if accel.shape[0] != particles.shape[0]:
+        raise ValueError("accel must have one row per particle")
+    charges = np.ascontiguousarray(charges, dtype=np.float32).reshape(-1, 4)
+    bmin = (ctypes.c_float * 3)(*[float(v) for v in box_min])
+    bmax = (ctypes.c_float * 3)(*[float(v) for v in box_max])
+    perf = em_perf()
+    call("em_verlet_step_f32", _np_ptr(particles), particles.shape[0],
+         _np_ptr(charges) if charges.size else None, charges.shape[0], _np_ptr(accel), bmin, bmax,
+         float(dt), int(bool(write_back)), ctypes.byref(perf))
+    return perf.as_dict()
+
+
 # ---------------------------------------------------------------------------
 # device tensors
 def _torch():
@@ -206,6 +227,26 @@ def trace64(charges, seeds, length: int, rule: StepRule, with_steps: bool = Fals
          rows[1].data_ptr(), rows[2].data_ptr(), n, length, n, ctypes.byref(st),
          steps.data_ptr() if steps is not None else None, _stream_handle())
     return (rows, steps) if with_steps else rows
+
+
+def verlet_step_dev(particles, charges, accel, box_min, box_max, dt: float, out=None):
+    """Device form of verlet_step: particles (n, 8), charges (p, 4), accel (n, 3)
+    float32 tensors; accel is updated in place; ``out`` (n, 8), if given, receives
+    the advanced particles (it must not be ``particles``)."""
+    torch = _torch()
+    _dev_check(particles, "particles", torch.float32)
+    _dev_check(accel, "accel", torch.float32)
+    if charges.numel():
+        _dev_check(charges, "charges", torch.float32)
+    if out is not None:
+        _dev_check(out, "out", torch.float32)
+    bmin = (ctypes.c_float * 3)(*[float(v) for v in box_min])
+    bmax = (ctypes.c_float * 3)(*[float(v) for v in box_max])
+    call("em_verlet_step_dev_f32", particles.data_ptr(), particles.shape[0],
+         charges.data_ptr() if charges.numel() else None, charges.shape[0] if charges.numel() else 0,
+         accel.data_ptr(), bmin, bmax, float(dt), out.data_ptr() if out is not None else None,
+         _stream_handle())
+    return accel
 
 
 def field(charges, points, scale: float = 1.0, launch=None):

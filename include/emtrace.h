@@ -200,6 +200,34 @@ int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x,
                      uint32_t ckpt_every, const float* d_gx, const float* d_gy,
                      const float* d_gz, float* d_grad_charges, void* stream);
 
+/* ---- particle step (the N-body client of the Coulomb sum) ------------------
+ * Replaces: emgVerletStep<float>(dynCharges, pointCharges, verletAccel,
+ *   minBound, maxBound, timeStep, n, perfData)  (ElectroMag/src/
+ *   Particle_System.cpp:108-190, never instantiated by the reference).
+ * particles : n records of 8 floats {x, y, z, q, vx, vy, vz, m}
+ *             (electro::dynamicPointCharge<float>, include/Electrodynamics.h)
+ * charges   : p static records {x, y, z, q}
+ * accel     : n x 3 floats, a(t) in, a(t + dt) out
+ * For each particle: v' = v + a dt/2, x' = x + v' dt, then with the OTHER
+ * particles at their old state
+ *   a = (q E(x') + q v' x B(x')) / m,
+ *   E = k_e [sum_static q_c r/|r|^3 + sum_j q_j r/|r|^3],
+ *   B = 1e-7 sum_j q_j v_j x r/|r|^3,   r = x' - source.
+ * The reference stores only a (it never writes the particle back).  With
+ * write_back (host form) or a non-NULL d_particles_out (device form, must not
+ * alias the input) the step also writes x' and v' + a dt/2, each reflected
+ * into [box_min, box_max] as BoundToBox does (Particle_System.cpp:19-56).
+ * box_min / box_max are host float[3].  Status: 1 when n == 0 (the
+ * reference's only check), 16+ as elsewhere. */
+int em_verlet_step_f32(float* particles, size_t n, const float* charges,
+                       size_t p, float* accel, const float* box_min,
+                       const float* box_max, float dt, int write_back,
+                       em_perf* perf);
+int em_verlet_step_dev_f32(const float* d_particles, size_t n,
+                           const float* d_charges, size_t p, float* d_accel,
+                           const float* box_min, const float* box_max,
+                           float dt, float* d_particles_out, void* stream);
+
 /* ---- the reference application's problem generator -----------------------
  * Host-only.  Restates ElectroMag/src/Electromag utils.h:29-111 (libc rand()
  * under srand) with a built-in copy of glibc's rand(), so the reference's

@@ -77,6 +77,8 @@ def lib() -> ctypes.CDLL:
         L.o64_adjoint_path.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams),
                                        ctypes.c_double, P, P, P, P, P]
         L.o64_adjoint_path.restype = ctypes.c_int
+        L.o64_verlet_step.argtypes = [P, S, P, S, P, P, P, ctypes.c_double, P]
+        L.o64_verlet_step.restype = None
         L.o64_min_dist_f32.argtypes = [P, S, P, S, P]
         L.o64_path_dist_f32.argtypes = [P, P, S, S, P, ctypes.c_longlong, P]
         for f in ("orc_calc_lines_f32", "orc_calc_lines_f64", "o64_trace", "o64_adjoint"):
@@ -232,6 +234,22 @@ def trace64(charges, seeds, length, rule, scale=None, with_steps=False):
     if rc:
         raise ValueError(f"o64_trace rc={rc}")
     return (rows, steps) if with_steps else rows
+
+
+def verlet64(particles, charges, accel, box_min, box_max, dt, write_back=False):
+    """FP64 particle step (Particle_System.cpp:108-190).  particles (n, 8)
+    {x, y, z, q, vx, vy, vz, m}, charges (p, 4), accel (n, 3) = a(t).
+    Returns (a(t + dt) (n, 3), new particles (n, 8) or None)."""
+    parts = np.ascontiguousarray(particles, dtype=np.float64)
+    ch = np.ascontiguousarray(charges, dtype=np.float64).reshape(-1, 4)
+    acc = np.array(accel, dtype=np.float64, order="C").reshape(-1, 3)
+    bmin = np.ascontiguousarray(box_min, dtype=np.float64)
+    bmax = np.ascontiguousarray(box_max, dtype=np.float64)
+    out = np.zeros_like(parts) if write_back else None
+    lib().o64_verlet_step(_ptr(parts), parts.shape[0], _ptr(ch) if ch.size else None, ch.shape[0],
+                          _ptr(acc), _ptr(bmin), _ptr(bmax), float(dt),
+                          _ptr(out) if out is not None else None)
+    return acc, out
 
 
 def adjoint64(charges, seeds, length, rule, G, scale=None):

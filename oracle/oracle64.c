@@ -18,6 +18,10 @@
  *                  No reference counterpart (the reference has no adjoint); it
  *                  is pinned by central finite differences of o64_trace in
  *                  tests/test_oracle.py.
+ *   o64_verlet_step the particle step of Particle_System.cpp:108-190 (new
+ *                  accelerations, optionally the written-back state), pinned
+ *                  by closed forms (Coulomb and Biot-Savart pairs, box
+ *                  reflection, Newton's third law) in tests/test_oracle.py.
  *
  * Parity pins: single-charge closed form (|E| = |q|/r^2, radial), linearity
  * in q, the divergence of J being zero away from charges (trace of the
@@ -361,4 +365,74 @@ int o64_adjoint_path(const double* ch, size_t p, const double* seeds, size_t n, 
                      double* gseed, const float* path, const unsigned* path_steps)
 {
     return adjoint_impl(ch, p, seeds, n, len, sp, scale, G, gch, gseed, path, path_steps);
+}
+
+/* ---------------------------------------------------------------------------
+ * Particle step, FP64 -- ElectroMag/src/Particle_System.cpp:108-190 read as its
+ * evident algorithm (the reference never instantiates it, and as written it
+ * does not compile: magneto_k is "1E-7;" and there is no scalar * vector or
+ * vector /= scalar operator).  Per particle i, the others at their old state:
+ *   v' = v + a dt/2;  x' = x + v' dt                         (134-140)
+ *   E  = sum_static r k_e q/(|r|^2 sqrt|r|^2)                 (143-148, Electrostatics.h:53-60)
+ *      + sum_{j != i} (k_e q_j) r/(|r|^2 sqrt|r|^2)            (PartFieldOp, Electrostatics.h:66-75)
+ *   B  = sum_{j != i} (q_j 1e-7) v_j x r/(|r|^2 sqrt|r|^2)     (Magnetics.h PartFieldOp)
+ *   a  = (q E + q (v' x B)) / m                              (173-178, Magnetics.h Force)
+ *   accel[i] = a;  with `out`: v' + a dt/2 and x', reflected into the box
+ *   (BoundToBox, 19-56), are written to out[i] (the reference discards them).
+ * Operation order is that of the reference's vec3 helpers.
+ * ------------------------------------------------------------------------- */
+void o64_verlet_step(const double* parts, size_t n, const double* charges, size_t p,
+                     double* accel, const double* bmin, const double* bmax, double dt,
+                     double* out)
+{
+    const double ke = 8.987551787E9, mk = 1E-7, dt2 = dt / 2;
+    long long nl = (long long)n;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < nl; ++i) {
+        const double* P = parts + 8 * i;
+        double vx = P[4], vy = P[5], vz = P[6];
+        const double q = P[3], m = P[7];
+        vx = vx + accel[3 * i] * dt2;
+        vy = vy + accel[3 * i + 1] * dt2;
+        vz = vz + accel[3 * i + 2] * dt2;
+        double x = P[0] + vx * dt, y = P[1] + vy * dt, z = P[2] + vz * dt;
+        double ex = 0, ey = 0, ez = 0, bx = 0, by = 0, bz = 0;
+        for (size_t j = 0; j < p; ++j) {
+            const double* c = charges + 4 * j;
+            double rx = x - c[0], ry = y - c[1], rz = z - c[2];
+            double l2 = rx * rx + ry * ry + rz * rz;
+            double s = ke * c[3] / (l2 * sqrt(l2));
+            ex += rx * s; ey += ry * s; ez += rz * s;
+        }
+        for (size_t j = 0; j < n; ++j) {
+            if ((long long)j == i) continue;
+            const double* S = parts + 8 * j;
+            double rx = x - S[0], ry = y - S[1], rz = z - S[2];
+            double l2 = rx * rx + ry * ry + rz * rz;
+            double d = l2 * sqrt(l2);
+            double ux = rx / d, uy = ry / d, uz = rz / d;     /* vec3InverseSquare */
+            double kq = ke * S[3];
+            ex += kq * ux; ey += kq * uy; ez += kq * uz;
+            double qm = S[3] * mk;                          /* q * magneto_k * cross(v, u) */
+            bx += qm * (S[5] * uz - S[6] * uy);
+            by += qm * (S[6] * ux - S[4] * uz);
+            bz += qm * (S[4] * uy - S[5] * ux);
+        }
+        double ax = q * ex + q * (vy * bz - vz * by);
+        double ay = q * ey + q * (vz * bx - vx * bz);
+        double az = q * ez + q * (vx * by - vy * bx);
+        ax = ax / m; ay = ay / m; az = az / m;
+        accel[3 * i] = ax; accel[3 * i + 1] = ay; accel[3 * i + 2] = az;
+        if (out) {
+            double wx = vx + ax * dt2, wy = vy + ay * dt2, wz = vz + az * dt2;
+            if (x > bmax[0]) { x = 2 * bmax[0] - x; wx = -wx; }
+            if (y > bmax[1]) { y = 2 * bmax[1] - y; wy = -wy; }
+            if (z > bmax[2]) { z = 2 * bmax[2] - z; wz = -wz; }
+            if (x < bmin[0]) { x = 2 * bmin[0] - x; wx = -wx; }
+            if (y < bmin[1]) { y = 2 * bmin[1] - y; wy = -wy; }
+            if (z < bmin[2]) { z = 2 * bmin[2] - z; wz = -wz; }
+            double* O = out + 8 * i;
+            O[0] = x; O[1] = y; O[2] = z; O[3] = q; O[4] = wx; O[5] = wy; O[6] = wz; O[7] = m;
+        }
+    }
 }

@@ -264,3 +264,61 @@ def test_generator_is_pinned():
 
 PINNED_CFG0 = "b5fe415cbbb4bf85"
 PINNED_CFG1 = "885a25643f9487f7"
+
+
+# ---------------------------------------------------------------------------
+# particle step (Particle_System.cpp:108-190): closed forms computed here
+# independently of oracle64.c
+KE_ = 8.987551787e9
+
+
+def _kick_drift(P, a, dt):
+    v = P[:, 4:7] + a * (dt / 2)
+    return P[:, 0:3] + v * dt, v
+
+
+def test_verlet64_coulomb_pair_with_drift(orc):
+    P = np.array([[0.1, -0.2, 0.3, 1e-3, 0.5, 0.0, -0.2, 2.0],
+                  [1.0, 0.5, -0.4, -2e-3, 0.0, 0.3, 0.1, 3.0]])
+    a_in = np.array([[1.0, 2.0, 0.0], [0.0, -1.0, 0.5]])
+    dt = 0.01
+    a, _ = orc.verlet64(P, np.zeros((0, 4)), a_in, [-9] * 3, [9] * 3, dt)
+    x, v = _kick_drift(P, a_in, dt)
+    for i, j in ((0, 1), (1, 0)):
+        r = x[i] - P[j, :3]  # the other particle at its OLD position
+        r3 = np.linalg.norm(r) ** 3
+        E = KE_ * P[j, 3] * r / r3
+        B = 1e-7 * P[j, 3] * np.cross(P[j, 4:7], r) / r3  # Biot-Savart, point charge
+        want = P[i, 3] * (E + np.cross(v[i], B)) / P[i, 7]
+        assert np.allclose(a[i], want, rtol=1e-12, atol=0), (i, a[i], want)
+
+
+def test_verlet64_static_charges_and_self_exclusion(orc):
+    rng = np.random.default_rng(3)
+    ch = np.concatenate([rng.uniform(-5, 5, (7, 3)), rng.uniform(-1, 1, (7, 1))], axis=1)
+    P = np.array([[0.3, 0.2, -0.1, 0.5, 0.1, 0.2, 0.3, 1.5]])
+    a, _ = orc.verlet64(P, ch, np.zeros((1, 3)), [-9] * 3, [9] * 3, 0.05)
+    x, _ = _kick_drift(P, np.zeros((1, 3)), 0.05)
+    r = x[0] - ch[:, :3]
+    E = KE_ * (ch[:, 3:4] * r / np.linalg.norm(r, axis=1, keepdims=True) ** 3).sum(axis=0)
+    assert np.allclose(a[0], P[0, 3] * E / P[0, 7], rtol=1e-12)
+    a0, _ = orc.verlet64(P, np.zeros((0, 4)), np.zeros((1, 3)), [-9] * 3, [9] * 3, 0.05)
+    assert np.array_equal(a0, np.zeros((1, 3)))  # a lone particle feels nothing of itself
+
+
+def test_verlet64_newton_third_law_and_box(orc):
+    rng = np.random.default_rng(5)
+    n = 40
+    P = np.zeros((n, 8))
+    P[:, :3] = rng.uniform(-2, 2, (n, 3))
+    P[:, 3] = rng.uniform(-1e-3, 1e-3, n)
+    P[:, 7] = rng.uniform(1, 2, n)
+    a, _ = orc.verlet64(P, np.zeros((0, 4)), np.zeros((n, 3)), [-9] * 3, [9] * 3, 0.0)
+    F = a * P[:, 7:8]
+    assert np.abs(F.sum(axis=0)).max() <= 1e-9 * np.abs(F).sum()
+    # box: a particle that drifts past the upper x bound is reflected
+    Q = np.array([[0.95, 0.0, 0.0, 0.0, 10.0, 0.0, 0.0, 1.0]])
+    a, out = orc.verlet64(Q, np.zeros((0, 4)), np.zeros((1, 3)), [-1] * 3, [1] * 3, 0.01, True)
+    assert np.allclose(out[0, :3], [2 * 1 - 1.05, 0, 0]) and np.allclose(out[0, 4:7], [-10, 0, 0])
+    a, none = orc.verlet64(Q, np.zeros((0, 4)), np.zeros((1, 3)), [-1] * 3, [1] * 3, 0.01, False)
+    assert none is None
