@@ -18,7 +18,9 @@
 //                        sources (a second N-body pass): per charge
 //                          dL/dq = sum w.d/r^3,
 //                          dL/dc = -q sum (w/r^3 - 3 d (d.w)/r^5),
-//                        one partial per (charge, source chunk);
+//                        one partial per (charge, source chunk); sources with
+//                        w = 0 (the steps after a line ended) are dropped
+//                        while a tile is staged;
 //   k_adjoint_reduce     sums the chunks in a fixed order (deterministic).
 #include <algorithm>
 #include <cmath>
@@ -29,12 +31,17 @@
 
 namespace em {
 
+// A chunk of the charge pass: the source tiles src0, src0 + stride,
+// src0 + 2 stride, ... (each kSrcTile sources) of one configuration, for one
+// block of charges.  Interleaving the tiles of the chunks spreads the sources
+// that matter (with the stop rule, mostly the early steps) over every chunk.
 struct ChargeItem {
     uint32_t cfg;
-    uint32_t charge0;  // first charge of the block
-    uint32_t src0;     // first source (index into the config's n*(len-1) sources)
-    uint32_t src_cnt;  // sources in this chunk
-    uint64_t part;     // first partial (float4) of this item: part + ci
+    uint32_t charge0;     // first charge of the block
+    uint64_t src0;        // first source of the chunk's first tile
+    uint64_t src_end;     // sources of the configuration: n * (len-1)
+    uint64_t src_stride;  // distance between the chunk's tiles
+    uint64_t part;        // first partial (float4) of this item: part + ci
 };
 
 struct RedItem {
@@ -163,6 +170,27 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
         const float4* ck = reinterpret_cast<const float4*>(ckpt + cfg.ckpt_off) +
                            2ull * ((uint64_t)m * cfg.n + li);
         const float4 c0 = ck[0], c1 = ck[1];
+        // No line of the wave (block, when the charges are re-staged per tile)
+        // is alive at the segment's checkpoint: nothing in it moves.  The sweep
+        // only carries dL/dx through the rows (x_s = x_{s-1}) and writes w = 0,
+        // the same operations the full path performs for an ended line.
+        const bool alive0 = valid && c1.w != 0.f;
+        if (!(single ? __any(alive0) : __syncthreads_or(alive0))) {
+            for (uint32_t s = s1; s > s0; --s) {
+                const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + li;
+                lx += GX[o];
+                ly += GY[o];
+                lz += GZ[o];
+                if (writer) {
+                    WX[o] = 0.f;
+                    WY[o] = 0.f;
+                    WZ[o] = 0.f;
+                }
+            }
+            lux = luy = luz = 0.f;
+            lh = 0.f;
+            continue;
+        }
         LineState st;
         st.px = c0.x; st.py = c0.y; st.pz = c0.z; st.h = c0.w;
         st.ux = c1.x; st.uy = c1.y; st.uz = c1.z;
@@ -310,8 +338,9 @@ struct ChargeBody {
 };
 
 constexpr uint32_t kSrcTile = 2048;  // sources per LDS tile (2 float4 each: 64 KiB)
+constexpr uint32_t kChargeBlock = 256;  // charges (threads) per workgroup of the charge pass
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kChargeBlock)
 k_adjoint_charges(const Cfg* __restrict__ cfgs, const ChargeItem* __restrict__ items,
                   const float4* __restrict__ charges, const float* __restrict__ X,
                   const float* __restrict__ Y, const float* __restrict__ Z,
@@ -327,26 +356,70 @@ k_adjoint_charges(const Cfg* __restrict__ cfgs, const ChargeItem* __restrict__ i
     ChargeBody body;
     body.init(c.x, c.y, c.z);
     const uint32_t n = cfg.n;
-    for (uint32_t t0 = 0; t0 < it.src_cnt; t0 += kSrcTile) {
-        const uint32_t cnt = min(kSrcTile, it.src_cnt - t0);
-        const uint32_t cntp = round_up(cnt, kGroup);
-        __syncthreads();
-        for (uint32_t j = threadIdx.x; j < cntp; j += blockDim.x) {
-            float4 xr = make_float4(kFar, kFar, kFar, 0.f), wr = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (j < cnt) {
-                const uint32_t g = it.src0 + t0 + j;  // source index: (s-1)*n + line
-                const uint32_t s = g / n + 1;
-                const uint32_t line = g - (s - 1) * n;
-                const uint64_t ox = cfg.line_off + (uint64_t)(s - 1) * cfg.pitch + line;
-                const uint64_t ow = cfg.line_off + (uint64_t)s * cfg.pitch + line;
-                xr = make_float4(X[ox], Y[ox], Z[ox], 0.f);
+    // Sources with w = 0 (every step after a line has ended) contribute
+    // nothing: each tile is filled with the next non-zero sources of the chunk,
+    // in source order, 256 candidates at a time (a ballot and a per-wave count
+    // give each kept source its slot).
+    __shared__ uint32_t wave_kept[kChargeBlock / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t base = it.src0;  // first source of the current source tile
+    uint32_t off = 0;         // candidates of that tile already looked at
+    uint64_t s0 = base / n;   // source index g = (s-1)*n + line: row of `base`
+    uint32_t l0 = (uint32_t)(base - s0 * n);
+    for (;;) {
+        __syncthreads();  // the previous tile has been consumed
+        uint32_t filled = 0;
+        while (filled + kChargeBlock <= kSrcTile && base < it.src_end) {
+            const uint32_t t = l0 + off + threadIdx.x;  // < n + kSrcTile
+            const uint32_t wraps = t / n;
+            const uint64_t g = base + off + threadIdx.x;
+            float4 xr, wr;
+            bool keep = false;
+            if (g < it.src_end) {
+                const uint64_t s = s0 + wraps + 1;
+                const uint32_t line = t - wraps * n;
+                const uint64_t ow = cfg.line_off + s * cfg.pitch + line;
                 wr = make_float4(WX[ow], WY[ow], WZ[ow], 0.f);
+                keep = wr.x != 0.f || wr.y != 0.f || wr.z != 0.f;
+                if (keep) {
+                    const uint64_t ox = ow - cfg.pitch;
+                    xr = make_float4(X[ox], Y[ox], Z[ox], 0.f);
+                }
             }
-            tile[2 * j] = xr;
-            tile[2 * j + 1] = wr;
+            const uint64_t kept = __ballot(keep);
+            if (lane == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
+            __syncthreads();
+            uint32_t slot = filled, total = 0;  // this wave's first tile slot; kept this round
+#pragma unroll
+            for (uint32_t v = 0; v < kChargeBlock / 64; ++v) {
+                const uint32_t kv = wave_kept[v];
+                slot += v < wave ? kv : 0u;
+                total += kv;
+            }
+            if (keep) {
+                const uint32_t at = slot + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
+                tile[2 * at] = xr;
+                tile[2 * at + 1] = wr;
+            }
+            filled += total;
+            off += kChargeBlock;
+            if (off == kSrcTile) {  // on to the chunk's next source tile
+                off = 0;
+                base += it.src_stride;
+                s0 = base / n;
+                l0 = (uint32_t)(base - s0 * n);
+            }
+            __syncthreads();  // wave_kept is rewritten next round
+        }
+        if (filled == 0) break;
+        const uint32_t cntp = round_up(filled, kGroup);
+        for (uint32_t j = filled + threadIdx.x; j < cntp; j += blockDim.x) {
+            tile[2 * j] = make_float4(kFar, kFar, kFar, 0.f);
+            tile[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
         pipelined_tile<1, 2>(tile, cntp, 0, body);
+        if (base >= it.src_end) break;
     }
     if (valid) {
         const float Hx = (body.Wx + body.cWx) - 3.f * (body.Dx + body.cDx);
@@ -483,15 +556,17 @@ int adj_prepare(AdjPlan* a, const std::vector<Cfg>& cfgs, const Shape& sh, uint3
         if (!c.p || !src) continue;
         double want = per_item / std::min<double>(c.p, 256.0);
         uint64_t chunk = (uint64_t)std::min<double>(std::max(2048.0, want), 4194304.0);
-        chunk = (chunk + 2047) / 2048 * 2048;
-        const uint32_t chunks = (uint32_t)((src + chunk - 1) / chunk);
-        for (uint32_t q0 = 0; q0 < c.p; q0 += 256) {
+        chunk = (chunk + kSrcTile - 1) / kSrcTile * kSrcTile;
+        const uint64_t tiles = (src + kSrcTile - 1) / kSrcTile;
+        const uint32_t chunks = (uint32_t)std::min<uint64_t>((src + chunk - 1) / chunk, tiles);
+        for (uint32_t q0 = 0; q0 < c.p; q0 += kChargeBlock) {
             for (uint32_t k = 0; k < chunks; ++k) {
                 ChargeItem it;
                 it.cfg = (uint32_t)i;
                 it.charge0 = q0;
-                it.src0 = (uint32_t)(k * chunk);
-                it.src_cnt = (uint32_t)std::min<uint64_t>(chunk, src - k * chunk);
+                it.src0 = (uint64_t)k * kSrcTile;
+                it.src_end = src;
+                it.src_stride = (uint64_t)chunks * kSrcTile;
                 it.part = part + (uint64_t)k * c.p + q0;
                 a->charge_items.push_back(it);
             }
@@ -546,7 +621,7 @@ int adj_launch(AdjPlan* a, const Cfg* d_cfgs, const float4* charges, const float
                               wx, wy, wz, gseed, stream));
     if (!a->charge_items.empty()) {
         const size_t lds = (size_t)kSrcTile * 2 * sizeof(float4);
-        hipLaunchKernelGGL(k_adjoint_charges, dim3((uint32_t)a->charge_items.size()), dim3(256), lds,
+        hipLaunchKernelGGL(k_adjoint_charges, dim3((uint32_t)a->charge_items.size()), dim3(kChargeBlock), lds,
                            stream, d_cfgs, a->d_charge_items, charges, x, y, z, wx, wy, wz,
                            a->d_part);
         EM_HIP(hipGetLastError());

@@ -24,16 +24,34 @@ from .rules import StepRule, angle
 class InverseFit:
     def __init__(self, workloads, target_charges, rule: StepRule = angle(), ckpt_every: int = 50,
                  lr_pos: float = 5e-3, lr_q: float = 2.5e-3, betas=(0.9, 0.999), eps: float = 1e-12,
-                 launch=None):
+                 launch=None, order_lines: bool = True):
         import torch
         self.torch = torch
         self.rule = rule
         self.every = ckpt_every
         self.layout, charges, self.lines = api.pack_batch(workloads)
+        if launch is None:
+            # With line ends most lines are short and the few full-length lines
+            # set the time of each pass: give every line as many lanes as the
+            # largest configuration keeps busy (>= 32 charges per lane, <= 16).
+            S = 1
+            while S < 16 and int(self.layout.p.max()) // (2 * S) >= 32:
+                S *= 2
+            launch = api.make_launch(S, 0, 256)
         self.theta = charges.clone()                       # parameters (sum p, 4)
         self.theta0 = charges.clone()
         self.true = torch.from_numpy(np.concatenate(target_charges)).cuda()
         self.plan = api.BatchPlan(self.layout, rule, launch, use_graph=False)
+        # Lines that end early (the stop rule) idle their lanes while a wave
+        # still holds a live line.  The loss is a sum over lines, so each
+        # configuration's lines are put in order of their step count at the
+        # starting charges (longest first, stable): waves then hold lines of
+        # similar length, and the forward and adjoint skip waves whose lines have
+        # all ended.  self.order[i][k] is the original index of line k of
+        # configuration i.
+        self.order = [np.arange(int(n)) for n in self.layout.n]
+        if order_lines and rule.stop:
+            self._order_lines()
         # targets: the same seeds traced through the true charges
         self.target = self.lines.clone()
         self.plan.run(self.true, self.target)
@@ -53,6 +71,23 @@ class InverseFit:
         self.lr = torch.tensor([lr_pos, lr_pos, lr_pos, lr_q], dtype=torch.float32, device="cuda")
         self.b1, self.b2, self.eps = float(betas[0]), float(betas[1]), float(eps)
         self.graph = None
+
+    def _order_lines(self):
+        torch = self.torch
+        steps = torch.empty(int(self.layout.n.sum()), dtype=torch.int32, device="cuda")
+        probe = self.lines.clone()
+        self.plan.run(self.theta, probe, steps)
+        steps = steps.cpu().numpy()
+        del probe
+        o = 0
+        for i in range(len(self.layout.n)):
+            n = int(self.layout.n[i])
+            perm = np.argsort(-steps[o:o + n].astype(np.int64), kind="stable")
+            o += n
+            view = self.layout.rows_view(self.lines, i)
+            idx = torch.from_numpy(perm).cuda()
+            view[:, 0, :] = view[:, 0, :].index_select(1, idx)
+            self.order[i] = perm
 
     # one iteration, as a sequence of launches on the current stream
     def _body(self):

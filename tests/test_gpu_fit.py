@@ -6,7 +6,9 @@
   workload agree with the FP64 oracle: the FP64 sweep along the GPU's own path
   (all lines), and the FP64 adjoint of the FP64 trajectory on the lines that
   follow it;
-* a short fit lowers the loss.
+* a short fit lowers the loss;
+* the fit orders each configuration's lines by step count (longest first) and
+  the order is a permutation of the seeds.
 """
 import numpy as np
 import pytest
@@ -69,17 +71,19 @@ def test_config4_subset_gradients_vs_fp64(gpu, orc):
         st = steps[off_line:off_line + w.n]
         off_line += w.n
         G64 = np.ascontiguousarray(g_rows.transpose(1, 2, 0))
-        gp, _ = orc.adjoint64_path(w.charges, w.seeds, w.len, rules.angle(), G64, rows, st)
+        seeds = np.ascontiguousarray(rows[:, 0, :].T)  # the fit's own line order
+        assert np.array_equal(np.sort(seeds, axis=0), np.sort(w.seeds, axis=0))
+        gp, _ = orc.adjoint64_path(w.charges, seeds, w.len, rules.angle(), G64, rows, st)
         o = int(fit.layout.charge_off[i])
         gg = grad[o:o + w.p]
         r_path = np.linalg.norm(gg - gp) / np.linalg.norm(gp)
         # own FP64 trajectory, on the lines that follow it
-        r64, s64 = orc.trace64(w.charges, w.seeds, w.len, rules.angle(), with_steps=True)
+        r64, s64 = orc.trace64(w.charges, seeds, w.len, rules.angle(), with_steps=True)
         err = np.abs(rows.astype(np.float64) - rows64_to_soa(r64)).max(axis=(0, 1))
         ok = (st == s64.astype(np.int64)) & (err <= 1e-3)
         Gm = G64.copy()
         Gm[:, ~ok, :] = 0.0
-        g64, _ = orc.adjoint64(w.charges, w.seeds, w.len, rules.angle(), Gm)
+        g64, _ = orc.adjoint64(w.charges, seeds, w.len, rules.angle(), Gm)
         Gmf = np.ascontiguousarray(Gm.transpose(2, 0, 1)).astype(np.float32)
         rep.append((w.p, w.n, float(r_path), float(ok.mean()), g64, Gmf, i))
         print(f"[config4 subset] cfg {i}: p={w.p} n={w.n} rel(gpu, FP64 along gpu path) {r_path:.2e}; "
@@ -111,3 +115,18 @@ def test_short_fit_lowers_the_loss(gpu):
     rec = fit.recovery()
     print(f"[fit] loss {l0:.4g} -> {l1:.4g}; recovery {rec}", flush=True)
     assert l1 < l0
+
+
+def test_lines_ordered_by_step_count(gpu):
+    ws, truth = _problem(6, (64, 400), (64, 300), 120, 0.1, 8)
+    fit = InverseFit(ws, truth, rules.angle(), ckpt_every=20)
+    steps = torch.empty(int(fit.layout.n.sum()), dtype=torch.int32, device="cuda")
+    fit.plan.run(fit.theta, fit.lines, steps)
+    steps = steps.cpu().numpy()
+    o = 0
+    for i, w in enumerate(ws):
+        st = steps[o:o + w.n]
+        o += w.n
+        assert (np.diff(st.astype(np.int64)) <= 0).all(), i  # longest first
+        seeds = fit.layout.rows_view(fit.lines, i)[:, 0, :].t().cpu().numpy()
+        assert np.array_equal(seeds, w.seeds[fit.order[i]]), i

@@ -161,3 +161,65 @@ def test_cli_double_precision_cpu_equals_gpu(gpu, tmp_path):
     assert open(tmp_path / "regression.txt").read() == ""
     w = legacy.preset("bogo")
     assert lines[0].startswith(" CPUL [0][0] x: %g" % float(w.seeds[0, 0]))
+
+
+def test_full_config1_fp32_vs_fp64_kernel(gpu, orc):
+    """The headline workload's charges and all 65,536 lines, first 250 rows,
+    angle rule (all steps), on both kernels.  The FP32 kernel's distance to the
+    FP64 kernel's polylines is no larger than the CPU FP32 path's (256 sampled
+    lines; prefixes before the first approach within 0.05 of a charge)."""
+    from helpers import prefix_lengths
+    w = gen.config1()
+    length = 250
+    rule = rules.angle(stop=False)
+    q = torch.from_numpy(w.charges).cuda()
+    s = torch.from_numpy(w.seeds).cuda()
+    r32 = gpu.trace(q, s, length, rule).cpu().numpy()
+    r64 = gpu.trace64(q.double(), s.double(), length, _as_f32_params(rule)).cpu().numpy()
+    assert np.isfinite(r32).all() and np.isfinite(r64).all()
+    idx = np.arange(0, w.n, 256)
+    rc, c32 = orc.calc_lines(w.charges, w.seeds[idx], length, rule)
+    assert rc == 0
+    ref = np.ascontiguousarray(r64[:, :, idx]).astype(np.float32)
+    first, _ = prefix_lengths(ref, w.charges)
+    e_gpu = orc.path_dist(np.ascontiguousarray(r32[:, :, idx]), ref, first).max(axis=0)
+    e_cpu = orc.path_dist(c32, ref, first).max(axis=0)
+    # all lines: distance of the FP32 kernel's points to the FP64 kernel's polylines
+    # (pointwise distances measure the arc-length phase of the adaptive step instead)
+    r64f = r64.astype(np.float32)
+    first_all, _ = prefix_lengths(r64f, w.charges)
+    allerr = orc.path_dist(r32, r64f, first_all).max(axis=0)
+    q_ = lambda e, f: float(np.quantile(e, f))  # noqa: E731
+    print(f"[config1 250 rows vs FP64 kernel] sampled {idx.size}: gpu path median "
+          f"{q_(e_gpu, .5):.2e} p90 {q_(e_gpu, .9):.2e}; cpu path median {q_(e_cpu, .5):.2e} "
+          f"p90 {q_(e_cpu, .9):.2e}; all {w.n} lines: path median {q_(allerr, .5):.2e} "
+          f"p99 {q_(allerr, .99):.2e}", flush=True)
+    assert q_(e_gpu, 0.5) <= 2 * q_(e_cpu, 0.5) + 1e-6
+    assert q_(e_gpu, 0.9) <= 2 * q_(e_cpu, 0.9) + 1e-6
+
+
+def test_full_config1_field_vs_fp64(gpu, orc):
+    """Per-evaluation field with all 16,384 config-1 charges vs the FP64 sum."""
+    w = gen.config1()
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([rng.uniform(-55, 55, (3000, 3)),
+                          w.seeds[rng.integers(0, w.n, 1000)].astype(np.float64)]).astype(np.float32)
+    E = gpu.field(torch.from_numpy(w.charges).cuda(), torch.from_numpy(pts).cuda(), 1.0)
+    E = E.cpu().numpy().astype(np.float64)
+    E64 = orc.field64(w.charges.astype(np.float64), pts.astype(np.float64), 1.0)
+    c64 = w.charges.astype(np.float64)
+    mag = np.concatenate([
+        (np.abs(c64[None, :, 3]) / ((pts[k:k + 250].astype(np.float64)[:, None, :] - c64[None, :, :3])
+                                    ** 2).sum(axis=2)).sum(axis=1)
+        for k in range(0, len(pts), 250)])
+    n64 = np.linalg.norm(E64, axis=1)
+    rel = np.linalg.norm(E - E64, axis=1) / n64
+    ok = n64 >= 0.1 * mag
+    print(f"[config1 field] {ok.sum()} well-conditioned of {len(pts)}: max rel {rel[ok].max():.3g}, "
+          f"median {np.median(rel[ok]):.3g}; all max |dE|/sum|E_i| "
+          f"{(np.linalg.norm(E - E64, axis=1) / mag).max():.3g}", flush=True)
+    assert ok.sum() > 500
+    assert rel[ok].max() <= 1e-5
+    # where the sum cancels: one FP32 term q d rsq(r^2)^3 carries up to ~10 ulp
+    # (the rsq error is tripled by the cube), so |dE| <= 2e-6 sum|E_i|
+    assert (np.linalg.norm(E - E64, axis=1) / mag).max() <= 2e-6
