@@ -8,12 +8,12 @@
 // dL/d(seeds):
 //   k_adjoint_lines<S>   per line, segment by segment from the last: recompute
 //                        the segment from its checkpoint (bit-identical to the
-//                        forward: same field_at and step_update), then sweep it
-//                        backward through x_s = x_{s-1} + h u, the step rule and
-//                        u = E/|E|, apply the field-gradient JVP
-//                        (dE/dx) w = f sum_i q_i (w/r^3 - 3 d (d.w)/r^5)
-//                        (an N-body pass over the charges), and store
-//                        w_s = f * dL/dE_s for every step;
+//                        forward: the same field sum and step_update), keeping
+//                        at every x_{s-1} also S3 = sum q/r^3 and
+//                        T = sum q d d^T/r^5; then sweep it backward through
+//                        x_s = x_{s-1} + h u, the step rule and u = E/|E|, apply
+//                        the field-gradient JVP (dE/dx) w = f (S3 w - 3 T w), and
+//                        store w_s = f * dL/dE_s for every step;
 //   k_adjoint_charges    charges as targets, the (x_{s-1}, w_s) pairs as
 //                        sources (a second N-body pass): per charge
 //                          dL/dq = sum w.d/r^3,
@@ -54,16 +54,27 @@ struct RedItem {
 
 namespace {
 
-// ---- field-gradient JVP body: (dE/dx) w without the factor f -------------
-struct JvpBody {
-    float px, py, pz, wx, wy, wz;
-    float s3, gx, gy, gz;            // block partials: sum q/r^3, sum a q/r^5 d
-    float S3, cS3, Gx, cGx, Gy, cGy, Gz, cGz;
-    __device__ void init(float x, float y, float z, float a, float b, float c)
+// ---- recompute body: the forward's field plus what the JVP needs ----------
+// The field-gradient JVP at x is (dE/dx) w = f (S3 w - 3 T w) with
+// S3 = sum q/r^3 and T = sum q d d^T/r^5 (d = x - c).  The recompute already
+// evaluates every charge at x = x_{s-1}, so it accumulates S3 and the six
+// components of T next to E (11 more VALU per charge); the backward sweep then
+// applies them to w with a 3x3 product instead of a second pass over the
+// charges.  E's operations are exactly those of FieldBody (coulomb(), same
+// fold points), so the recomputed path equals the forward's bit for bit.
+struct FieldTensorBody {
+    float px, py, pz;
+    float ex, ey, ez, s3, txx, txy, txz, tyy, tyz, tzz;  // block partials
+    Acc acc;                                          // E, compensated
+    float S3, cS3, T[6], cT[6];                       // S3 and T, compensated
+    __device__ void init(float x, float y, float z)
     {
-        px = x; py = y; pz = z; wx = a; wy = b; wz = c;
-        s3 = gx = gy = gz = 0.f;
-        S3 = cS3 = Gx = cGx = Gy = cGy = Gz = cGz = 0.f;
+        px = x; py = y; pz = z;
+        ex = ey = ez = s3 = txx = txy = txz = tyy = tyz = tzz = 0.f;
+        acc.zero();
+        S3 = cS3 = 0.f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) T[i] = cT[i] = 0.f;
     }
     __device__ __forceinline__ void term(const float4* rec)
     {
@@ -74,32 +85,46 @@ struct JvpBody {
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, kSoft2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
         const float inv2 = inv * inv;
-        const float i3q = (c.w * inv) * inv2;
-        s3 += i3q;
-        const float a = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
-        const float b = (a * i3q) * inv2;
-        gx = __builtin_fmaf(b, dx, gx);
-        gy = __builtin_fmaf(b, dy, gy);
-        gz = __builtin_fmaf(b, dz, gz);
+        const float s = (c.w * inv) * inv2;  // coulomb(): (q inv)(inv inv)
+        ex = __builtin_fmaf(s, dx, ex);
+        ey = __builtin_fmaf(s, dy, ey);
+        ez = __builtin_fmaf(s, dz, ez);
+        s3 += s;
+        const float i5 = s * inv2;
+        const float ax = i5 * dx, ay = i5 * dy, az = i5 * dz;
+        txx = __builtin_fmaf(ax, dx, txx);
+        txy = __builtin_fmaf(ax, dy, txy);
+        txz = __builtin_fmaf(ax, dz, txz);
+        tyy = __builtin_fmaf(ay, dy, tyy);
+        tyz = __builtin_fmaf(ay, dz, tyz);
+        tzz = __builtin_fmaf(az, dz, tzz);
     }
     __device__ __forceinline__ void fold()
     {
+        em::fold(acc.sx, acc.cx, ex);
+        em::fold(acc.sy, acc.cy, ey);
+        em::fold(acc.sz, acc.cz, ez);
         em::fold(S3, cS3, s3);
-        em::fold(Gx, cGx, gx);
-        em::fold(Gy, cGy, gy);
-        em::fold(Gz, cGz, gz);
-        s3 = gx = gy = gz = 0.f;
+        em::fold(T[0], cT[0], txx);
+        em::fold(T[1], cT[1], txy);
+        em::fold(T[2], cT[2], txz);
+        em::fold(T[3], cT[3], tyy);
+        em::fold(T[4], cT[4], tyz);
+        em::fold(T[5], cT[5], tzz);
+        ex = ey = ez = s3 = txx = txy = txz = tyy = tyz = tzz = 0.f;
     }
 };
 
+// E (as field_at computes it), S3 and T at (px, py, pz).  Same staging and
+// barrier structure as field_at: every thread of the block calls it.
 template <int S>
-__device__ __forceinline__ void jvp_at(float4* __restrict__ tile, uint32_t tile_cap,
-                                       const float4* __restrict__ charges, uint32_t p, int k,
-                                       float px, float py, float pz, float wx, float wy, float wz,
-                                       float& jx, float& jy, float& jz)
+__device__ __forceinline__ void field_tensor_at(float4* __restrict__ tile, uint32_t tile_cap,
+                                                const float4* __restrict__ charges, uint32_t p,
+                                                int k, float px, float py, float pz, float e[3],
+                                                float& s3, float t[6])
 {
-    JvpBody body;
-    body.init(px, py, pz, wx, wy, wz);
+    FieldTensorBody body;
+    body.init(px, py, pz);
     constexpr uint32_t G = S * kGroup;
     const uint32_t pp = round_up(p, G);
     if (pp <= tile_cap) {
@@ -114,22 +139,24 @@ __device__ __forceinline__ void jvp_at(float4* __restrict__ tile, uint32_t tile_
             pipelined_tile<S, 1>(tile, cntp, k, body);
         }
     }
-    const float S3 = group_sum<S>(body.S3 + body.cS3);
-    const float Gx = group_sum<S>(body.Gx + body.cGx);
-    const float Gy = group_sum<S>(body.Gy + body.cGy);
-    const float Gz = group_sum<S>(body.Gz + body.cGz);
-    jx = wx * S3 - 3.f * Gx;
-    jy = wy * S3 - 3.f * Gy;
-    jz = wz * S3 - 3.f * Gz;
+    e[0] = group_sum<S>(body.acc.sx + body.acc.cx);
+    e[1] = group_sum<S>(body.acc.sy + body.acc.cy);
+    e[2] = group_sum<S>(body.acc.sz + body.acc.cz);
+    s3 = group_sum<S>(body.S3 + body.cS3);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) t[i] = group_sum<S>(body.T[i] + body.cT[i]);
 }
 
-// Per-lane recompute record: three float4 per step, coalesced across lanes:
-// rec[(j*3 + q) * lanes + lane].
+// Recompute record of a segment: five float4 per step, one set per line (the
+// S lanes of a line hold the same values; the group's first lane writes them
+// and all S read them): rec[(j*5 + q) * lanes + slot], slot = the group's first
+// lane, so a wave's accesses stay within one 1 KiB run.
+constexpr int kRec = 5;
 struct RecPtr {
     float4* base;
     uint64_t lanes;
-    uint64_t lane;
-    __device__ float4* at(uint32_t j, int q) const { return base + ((uint64_t)j * 3 + q) * lanes + lane; }
+    uint64_t slot;
+    __device__ float4* at(uint32_t j, int q) const { return base + ((uint64_t)j * kRec + q) * lanes + slot; }
 };
 
 template <int S>
@@ -158,7 +185,7 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
         stage(tile, ch, p, pp);
         __syncthreads();
     }
-    RecPtr rec{rec_base, (uint64_t)gridDim.x * blockDim.x, (uint64_t)blockIdx.x * blockDim.x + tid};
+    RecPtr rec{rec_base, (uint64_t)gridDim.x * blockDim.x, (uint64_t)blockIdx.x * blockDim.x + tid - k};
     const uint32_t steps = cfg.len - 1;
     const uint32_t nseg = (steps + every - 1) / every;
     float lx = 0.f, ly = 0.f, lz = 0.f;     // dL/dx carried backward
@@ -191,6 +218,13 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
             lh = 0.f;
             continue;
         }
+        // A wave entering live segment m still has m segments ahead; with line
+        // ends the few long lines set the pass's time, so their waves win the
+        // SIMD's issue arbitration over the many short ones (results unchanged).
+        if (4 * (uint32_t)m >= 3 * nseg) __builtin_amdgcn_s_setprio(3);
+        else if (2 * (uint32_t)m >= nseg) __builtin_amdgcn_s_setprio(2);
+        else if (4 * (uint32_t)m >= nseg) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
         LineState st;
         st.px = c0.x; st.py = c0.y; st.pz = c0.z; st.h = c0.w;
         st.ux = c1.x; st.uy = c1.y; st.uz = c1.z;
@@ -204,22 +238,26 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
             const float xx = st.px, xy = st.py, xz = st.pz;
             StepRec r;
             r.L = 1.f; r.c = 0.f; r.h = 0.f; r.branch = 4; r.ux = r.uy = r.uz = 0.f;
+            float s3 = 0.f, t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             const bool need = single ? __any(st.alive) : __syncthreads_or(st.alive);
             if (need) {
-                float ex, ey, ez;
-                field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
-                if (st.alive) {
-                    ex *= sp.scale;
-                    ey *= sp.scale;
-                    ez *= sp.scale;
-                    step_update(sp, s, ex, ey, ez, st, &r);
-                }
+                float e[3];
+                field_tensor_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, e, s3, t);
+                if (st.alive) step_update(sp, s, e[0] * sp.scale, e[1] * sp.scale, e[2] * sp.scale, st, &r);
             }
-            const uint32_t j = s - s0 - 1;
-            *rec.at(j, 0) = make_float4(xx, xy, xz, r.L);
-            *rec.at(j, 1) = make_float4(r.ux, r.uy, r.uz, r.h);
-            *rec.at(j, 2) = make_float4(r.c, (float)r.branch, 0.f, 0.f);
+            if (k == 0) {
+                const uint32_t j = s - s0 - 1;
+                *rec.at(j, 0) = make_float4(xx, xy, xz, r.L);
+                *rec.at(j, 1) = make_float4(r.ux, r.uy, r.uz, r.h);
+                *rec.at(j, 2) = make_float4(r.c, (float)r.branch, 0.f, 0.f);
+                *rec.at(j, 3) = make_float4(s3, t[0], t[1], t[2]);
+                *rec.at(j, 4) = make_float4(t[3], t[4], t[5], 0.f);
+            }
         }
+        // the record was written by each group's first lane and is read by all
+        // S lanes of the group (same wave): order those writes before the reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // backward sweep
         for (uint32_t s = s1; s > s0; --s) {
             const uint32_t j = s - s0 - 1;
@@ -267,15 +305,15 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
                 WY[o] = wy;
                 WZ[o] = wz;
             }
-            const bool need = single ? __any(moved) : __syncthreads_or(moved);
-            if (need) {
-                float jx, jy, jz;
-                jvp_at<S>(tile, tile_cap, ch, p, k, r0.x, r0.y, r0.z, wx, wy, wz, jx, jy, jz);
-                if (moved) {
-                    lx += jx;
-                    ly += jy;
-                    lz += jz;
-                }
+            if (moved) {
+                // (dE/dx) w = S3 w - 3 T w at x_{s-1} (f is inside w)
+                const float4 r3 = *rec.at(j, 3), r4 = *rec.at(j, 4);
+                const float tx = __builtin_fmaf(r3.w, wz, __builtin_fmaf(r3.z, wy, r3.y * wx));
+                const float ty = __builtin_fmaf(r4.y, wz, __builtin_fmaf(r4.x, wy, r3.z * wx));
+                const float tz = __builtin_fmaf(r4.z, wz, __builtin_fmaf(r4.y, wy, r3.w * wx));
+                lx += __builtin_fmaf(-3.f, tx, wx * r3.x);
+                ly += __builtin_fmaf(-3.f, ty, wy * r3.x);
+                lz += __builtin_fmaf(-3.f, tz, wz * r3.x);
             }
             lux = lpx;
             luy = lpy;
@@ -577,7 +615,7 @@ int adj_prepare(AdjPlan* a, const std::vector<Cfg>& cfgs, const Shape& sh, uint3
     a->w_plane = line_elems;
     const uint64_t lanes = (uint64_t)a->line_items.size() * sh.block;
     EM_TRY(grow(reinterpret_cast<void**>(&a->d_rec), &a->rec_bytes,
-                lanes * (size_t)every * 3 * sizeof(float4)));
+                lanes * (size_t)every * kRec * sizeof(float4)));
     size_t wb = 0;
     if (a->d_w) wb = 0;  // always re-size below
     if (a->d_w) EM_HIP(hipFree(a->d_w));
