@@ -203,7 +203,34 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
         // the same operations the full path performs for an ended line.
         const bool alive0 = valid && c1.w != 0.f;
         if (!(single ? __any(alive0) : __syncthreads_or(alive0))) {
-            for (uint32_t s = s1; s > s0; --s) {
+            // rows in blocks of kDeadRows: all loads of a block are issued
+            // before its (in-order) additions, so the walk is not one memory
+            // latency per row
+            constexpr uint32_t kDeadRows = 10;
+            uint32_t s = s1;
+            for (; s >= s0 + kDeadRows; s -= kDeadRows) {
+                float gx[kDeadRows], gy[kDeadRows], gz[kDeadRows];
+#pragma unroll
+                for (uint32_t r = 0; r < kDeadRows; ++r) {
+                    const uint64_t o = cfg.line_off + (uint64_t)(s - r) * cfg.pitch + li;
+                    gx[r] = GX[o];
+                    gy[r] = GY[o];
+                    gz[r] = GZ[o];
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < kDeadRows; ++r) {
+                    lx += gx[r];
+                    ly += gy[r];
+                    lz += gz[r];
+                    if (writer) {
+                        const uint64_t o = cfg.line_off + (uint64_t)(s - r) * cfg.pitch + li;
+                        WX[o] = 0.f;
+                        WY[o] = 0.f;
+                        WZ[o] = 0.f;
+                    }
+                }
+            }
+            for (; s > s0; --s) {
                 const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + li;
                 lx += GX[o];
                 ly += GY[o];
