@@ -232,10 +232,9 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     float* dz = dy + len * n;
 
     EM_HIP(hipEventRecord(b.ev[0], b.stream));
-    if (p) EM_HIP(hipMemcpyAsync(b.charges, charges, p * sizeof(float4), hipMemcpyHostToDevice, b.stream));
-    EM_HIP(hipMemcpyAsync(dx, x, row, hipMemcpyHostToDevice, b.stream));
-    EM_HIP(hipMemcpyAsync(dy, y, row, hipMemcpyHostToDevice, b.stream));
-    EM_HIP(hipMemcpyAsync(dz, z, row, hipMemcpyHostToDevice, b.stream));
+    const CopySeg in[4] = {{dx, x, row}, {dy, y, row}, {dz, z, row},
+                           {b.charges, charges, p * sizeof(float4)}};
+    EM_TRY(h2d_staged(dev.device, in, p ? 4 : 3, b.stream));
     EM_HIP(hipEventRecord(b.ev[1], b.stream));
 
     Cfg cfg{};
@@ -250,22 +249,21 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
                                nullptr, 1, dsteps, b.stream));
     EM_HIP(hipEventRecord(b.ev[2], b.stream));
+    EM_HIP(hipEventSynchronize(b.ev[2]));
+    const double td = now_s();
     const size_t out_bytes = (len - 1) * row;
-    EM_HIP(hipMemcpyAsync(x + n, dx + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
-    EM_HIP(hipMemcpyAsync(y + n, dy + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
-    EM_HIP(hipMemcpyAsync(z + n, dz + n, out_bytes, hipMemcpyDeviceToHost, b.stream));
-    if (steps) EM_HIP(hipMemcpyAsync(steps, dsteps, n * sizeof(uint32_t), hipMemcpyDeviceToHost, b.stream));
-    EM_HIP(hipEventRecord(b.ev[3], b.stream));
-    EM_HIP(hipEventSynchronize(b.ev[3]));
+    const CopySeg segs[4] = {{x + n, dx + n, out_bytes}, {y + n, dy + n, out_bytes},
+                             {z + n, dz + n, out_bytes}, {steps, dsteps, n * sizeof(uint32_t)}};
+    EM_TRY(d2h_staged(dev.device, segs, steps ? 4 : 3, b.stream));
+    const double d2h = now_s() - td;
 
-    float ms[3] = {0, 0, 0};
+    float ms[2] = {0, 0};
     EM_HIP(hipEventElapsedTime(&ms[0], b.ev[0], b.ev[1]));
     EM_HIP(hipEventElapsedTime(&ms[1], b.ev[1], b.ev[2]));
-    EM_HIP(hipEventElapsedTime(&ms[2], b.ev[2], b.ev[3]));
     if (perf) {
         perf->time_s = now_s() - t0;
         perf->h2d_s = ms[0] * 1e-3;
-        perf->d2h_s = ms[2] * 1e-3;
+        perf->d2h_s = d2h;
         perf->h2d_bytes = (double)(p * sizeof(float4) + 3 * row);
         perf->d2h_bytes = (double)(3 * out_bytes);
         fill_perf(perf, (double)p, (double)n, (double)len, ms[1] * 1e-3);
@@ -330,6 +328,7 @@ int em_shutdown(void)
         b = HostPathBuffers();
     }
     EM_HIP(hipSetDevice(cur));
+    EM_TRY(staging_release_all());
     return adj_release_all();
 }
 

@@ -77,6 +77,20 @@ int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float
                  const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,
                  float* ez, hipStream_t stream);
 
+// device -> pageable host copies through pinned staging buffers (staging.cpp);
+// issued on `stream` after the work that produces `src`, returns when every
+// byte is in `dst`
+struct CopySeg {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+int d2h_staged(int device, const CopySeg* segs, int nseg, hipStream_t stream);
+// host (pageable) -> device, through the same buffers; returns when the
+// copies have completed
+int h2d_staged(int device, const CopySeg* segs, int nseg, hipStream_t stream);
+int staging_release_all();
+
 // adjoint (adjoint.hip)
 struct AdjPlan;
 AdjPlan* adj_new();

@@ -270,15 +270,14 @@ int em_verlet_step_f32(float* particles, size_t n, const float* charges, size_t 
     float* dout = dp + n * 8;
     float* dq = dout + n * 8;
     float* da = dq + std::max<size_t>(p, 1) * 4;
-    EM_HIP(hipMemcpyAsync(dp, particles, pb, hipMemcpyHostToDevice, r.s));
-    if (p) EM_HIP(hipMemcpyAsync(dq, charges, p * 4 * sizeof(float), hipMemcpyHostToDevice, r.s));
-    EM_HIP(hipMemcpyAsync(da, accel, ab, hipMemcpyHostToDevice, r.s));
+    const CopySeg in[3] = {{dp, particles, pb}, {da, accel, ab}, {dq, charges, p * 4 * sizeof(float)}};
+    EM_TRY(h2d_staged(dev.device, in, p ? 3 : 2, r.s));
     EM_HIP(hipEventRecord(r.ev[0], r.s));
     EM_TRY(launch_verlet(dp, (uint32_t)n, dq, (uint32_t)p, da, make_vp(box_min, box_max, dt),
                          write_back ? dout : nullptr, r.s));
     EM_HIP(hipEventRecord(r.ev[1], r.s));
-    EM_HIP(hipMemcpyAsync(accel, da, ab, hipMemcpyDeviceToHost, r.s));
-    if (write_back) EM_HIP(hipMemcpyAsync(particles, dout, pb, hipMemcpyDeviceToHost, r.s));
+    const CopySeg segs[2] = {{accel, da, ab}, {particles, dout, pb}};
+    EM_TRY(d2h_staged(dev.device, segs, write_back ? 2 : 1, r.s));
     EM_HIP(hipStreamSynchronize(r.s));
     float ms = 0.f;
     EM_HIP(hipEventElapsedTime(&ms, r.ev[0], r.ev[1]));

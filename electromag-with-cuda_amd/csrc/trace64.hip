@@ -302,33 +302,30 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
     double* dz = dy + len * n;
 
     EM_HIP(hipEventRecord(r.ev[0], r.stream));
-    if (p) EM_HIP(hipMemcpyAsync(r.q, charges, p * sizeof(double4), hipMemcpyHostToDevice, r.stream));
-    EM_HIP(hipMemcpyAsync(dx, x, row, hipMemcpyHostToDevice, r.stream));
-    EM_HIP(hipMemcpyAsync(dy, y, row, hipMemcpyHostToDevice, r.stream));
-    EM_HIP(hipMemcpyAsync(dz, z, row, hipMemcpyHostToDevice, r.stream));
+    const CopySeg in[4] = {{dx, x, row}, {dy, y, row}, {dz, z, row},
+                           {r.q, charges, p * sizeof(double4)}};
+    EM_TRY(h2d_staged(dev.device, in, p ? 4 : 3, r.stream));
     EM_HIP(hipEventRecord(r.ev[1], r.stream));
     EM_TRY(launch64(static_cast<const double*>(r.q), (uint32_t)p, dx, dy, dz, (uint32_t)n,
                     (uint32_t)len, n, sp, static_cast<uint32_t*>(r.st), r.stream));
     EM_HIP(hipEventRecord(r.ev[2], r.stream));
+    EM_HIP(hipEventSynchronize(r.ev[2]));
+    const double td = now64();
     const size_t out_bytes = (len - 1) * row;
-    EM_HIP(hipMemcpyAsync(x + n, dx + n, out_bytes, hipMemcpyDeviceToHost, r.stream));
-    EM_HIP(hipMemcpyAsync(y + n, dy + n, out_bytes, hipMemcpyDeviceToHost, r.stream));
-    EM_HIP(hipMemcpyAsync(z + n, dz + n, out_bytes, hipMemcpyDeviceToHost, r.stream));
-    if (steps)
-        EM_HIP(hipMemcpyAsync(steps, r.st, n * sizeof(uint32_t), hipMemcpyDeviceToHost, r.stream));
-    EM_HIP(hipEventRecord(r.ev[3], r.stream));
-    EM_HIP(hipEventSynchronize(r.ev[3]));
+    const CopySeg segs[4] = {{x + n, dx + n, out_bytes}, {y + n, dy + n, out_bytes},
+                             {z + n, dz + n, out_bytes}, {steps, r.st, n * sizeof(uint32_t)}};
+    EM_TRY(d2h_staged(dev.device, segs, steps ? 4 : 3, r.stream));
+    const double d2h = now64() - td;
 
-    float ms[3] = {0, 0, 0};
+    float ms[2] = {0, 0};
     EM_HIP(hipEventElapsedTime(&ms[0], r.ev[0], r.ev[1]));
     EM_HIP(hipEventElapsedTime(&ms[1], r.ev[1], r.ev[2]));
-    EM_HIP(hipEventElapsedTime(&ms[2], r.ev[2], r.ev[3]));
     if (perf) {
         const double ks = ms[1] * 1e-3, dn = (double)n, dl = (double)len, dp = (double)p;
         perf->time_s = now64() - t0;
         perf->kernel_s = ks;
         perf->h2d_s = ms[0] * 1e-3;
-        perf->d2h_s = ms[2] * 1e-3;
+        perf->d2h_s = d2h;
         perf->h2d_bytes = (double)(p * sizeof(double4) + 3 * row);
         perf->d2h_bytes = (double)(3 * out_bytes);
         perf->interactions = dp * dn * (dl - 1.0);
