@@ -233,14 +233,19 @@ def extras(w, charges, seeds, plain_value, launch):
     res = {}
     # 1. same workload, lines end where they reverse (useful interactions only)
     rule_s = rules.angle(stop=True)
+    launch_s, rep_s = autotune.tune_plain(charges, seeds, rule_s)
     rows = emtrace.seed_rows(seeds, w.len)
     st = torch.empty(w.n, dtype=torch.int32, device="cuda")
-    wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule_s, launch, steps=st), 1, 1)
+    wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule_s, launch_s, steps=st), 1, 1)
     taken = float(st.double().sum().item())
     res["config1_with_line_ends"] = {
         "useful_interactions_per_s": w.p * taken / kern[0], "kernel_ms": kern[0] * 1e3,
-        "mean_steps": taken / w.n, "note": "lines end at sinks; waves whose lines all ended stop "
-        "evaluating, lines that ended inside a live wave idle their lanes"}
+        "mean_steps": taken / w.n,
+        "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
+                   "block": launch_s.block},
+        "note": "lines end at sinks; a wave (a block, when the charges are re-staged) stops "
+                "evaluating once all its lines have ended, so the tuned shape has few lines per "
+                "wave and one-wave blocks"}
     del rows
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
     ws = gen.config2()

@@ -742,7 +742,7 @@ extern "C" int em_adjoint_dev_f32(const float* d_charges, size_t p, const float*
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    EM_TRY(choose_shape(dev, p, n, launch, sh));
+    EM_TRY(choose_shape(dev, p, n, launch, sh, step->stop != 0));  // as the forward chose
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_adj_mu);
     AdjPlan* a = &g_adj[dev.device];

@@ -90,12 +90,21 @@ static uint32_t tile_for(const DeviceInfo& dev, uint64_t p, int S, int resident)
     return (uint32_t)std::min(t, tmax);
 }
 
-int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out)
+int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
+                 bool stop)
 {
     Shape s;
-    s.block = (req && req->block) ? req->block : 256;
+    // With line ends a wave keeps evaluating until its last line has ended (a
+    // whole block, when the charges are re-staged tile by tile): few lines per
+    // wave, and one-wave blocks over small tiles when the charges do not fit.
+    const bool restaged = (uint64_t)p * sizeof(float4) > dev.lds_per_block;
+    s.block = (req && req->block) ? req->block : (stop && restaged ? 64 : 256);
     if (req && req->lanes_per_line) {
         s.S = req->lanes_per_line;
+    } else if (stop) {
+        int S = 1;
+        while (S < 32 && p / (2ull * S) >= 64) S *= 2;
+        s.S = S;
     } else {
         const uint64_t target_waves = 2ull * 4ull * (uint64_t)dev.cus;  // two waves per SIMD
         int S = 1;
@@ -104,6 +113,8 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
     }
     if (req && req->tile) {
         s.tile = (uint32_t)req->tile;
+    } else if (stop && restaged && (uint32_t)(s.S * kGroup) <= 1024u) {
+        s.tile = 1024;
     } else {
         const uint64_t blocks = (n * (uint64_t)s.S + s.block - 1) / s.block;
         const int resident = (int)std::min<uint64_t>(std::max<uint64_t>(1, (blocks + dev.cus - 1) / dev.cus),
@@ -214,7 +225,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    EM_TRY(choose_shape(dev, p, n, launch, sh));
+    EM_TRY(choose_shape(dev, p, n, launch, sh, st->stop != 0));
 
     std::lock_guard<std::mutex> lk(g_mu);
     HostPathBuffers& b = g_hp[dev.device];
@@ -369,7 +380,7 @@ int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, f
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    EM_TRY(choose_shape(dev, p, n, launch, sh));
+    EM_TRY(choose_shape(dev, p, n, launch, sh, step->stop != 0));
     Cfg cfg{};
     cfg.pitch = pitch;
     cfg.p = (uint32_t)p;

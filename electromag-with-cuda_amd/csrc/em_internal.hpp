@@ -58,7 +58,10 @@ struct Shape {
 // Heuristic launch shape for one configuration (p charges, n lines); fields
 // set in `req` (non-zero) override it.  Fails with EM_ERR_INVALID on an
 // unsupported request.
-int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out);
+// `stop`: the angle rule with line ends (favours wide line groups and small
+// blocks); the forward and the adjoint of one problem must choose alike.
+int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
+                 bool stop = false);
 int validate_shape(const DeviceInfo& dev, const Shape& s);
 
 StepP to_stepp(const em_step& st);

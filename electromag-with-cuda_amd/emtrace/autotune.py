@@ -66,9 +66,10 @@ def _device_key() -> str:
 # local cache has no entry: the fastest shape of each sweep, ties (within
 # 1%) going to fewer lanes per line.  [S, block, tile, ms at 48 rows]
 SHIPPED = {
-    # 16,384 charges, 65,536 lines, angle rule (all steps / with line ends)
+    # 16,384 charges, 65,536 lines, angle rule (all steps / with line ends:
+    # 239 ms vs 917 ms for the all-steps shape)
     "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, None],
-    "plain:gfx950:256:16384:65536:2:1": [4, 512, 4096, None],
+    "plain:gfx950:256:16384:65536:2:1": [32, 64, 1024, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule
     "batch:gfx950:256:512:8192:1048576:2:0": [4, 256, 4096, None],
 }
@@ -94,25 +95,35 @@ def _round_up(a: int, b: int) -> int:
     return (a + b - 1) // b * b
 
 
-def candidates(p: int, n: int, cus: int, batch: bool = False) -> List[Tuple[int, int, int]]:
+def candidates(p: int, n: int, cus: int, batch: bool = False,
+               stop: bool = False) -> List[Tuple[int, int, int]]:
     """(S, block, tile) shapes worth timing for p charges and n lines.
 
     Single configuration: S grows until the grid has 8 waves per SIMD.  Batch:
     S in {1, 2, 4, 8} always -- a smaller line block per work item evens out
-    the persistent grid's finishing times even when waves are plentiful."""
+    the persistent grid's finishing times even when waves are plentiful.
+    With line ends (stop): S up to 32 and blocks down to one wave whatever the
+    line count -- a wave (with re-staged charges, a block) runs until its last
+    line ends, so fewer lines per wave and per block waste fewer lanes -- and
+    smaller tiles, which keep several one-wave blocks resident per CU."""
     out = []
     for S in (1, 2, 4, 8, 16, 32):
         if batch:
             if S > 8:
                 continue
+        elif stop:
+            if S < 4:
+                continue
         elif S > 1 and math.ceil(n * (S // 2) / 64) >= 8 * 4 * cus:
             continue  # already 8 waves per SIMD at S/2: splitting further only adds reductions
         if S > 1 and p / S < 32:
             continue
-        for block in (256, 512):
+        for block in ((64, 128, 256, 512) if stop and not batch else (256, 512)):
             q = max(256, S * GROUP)
             need = _round_up(max(p, 1), q)
             tiles = set()
+            if stop and not batch:
+                tiles.update(t for t in (1024, 2048, 4096) if t % q == 0 and t < need)
             for resident in (1, 2):
                 tmax = (LDS_BYTES // resident // 16) // q * q
                 if tmax < q:
@@ -143,20 +154,24 @@ def _time(fn, reps: int = 3) -> float:
     return best
 
 
-def tune_plain(charges, seeds, rule: StepRule, rows: int = 48, use_cache: bool = True,
+def tune_plain(charges, seeds, rule: StepRule, rows: Optional[int] = None, use_cache: bool = True,
                max_candidates: Optional[int] = None):
     """Best launch for tracing ``seeds`` through ``charges`` with ``rule``.
 
-    Returns (em_launch, report) where report lists (S, block, tile, ms)."""
+    Candidates are timed on ``rows`` rows (48; 400 with line ends, so that the
+    lines' different lengths show).  Returns (em_launch, report) where report
+    lists (S, block, tile, ms)."""
     import torch
     p, n = int(charges.shape[0]), int(seeds.shape[0])
+    if rows is None:
+        rows = 400 if rule.stop else 48
     key = f"plain:{_device_key()}:{_bucket(p)}:{_bucket(n)}:{rule.rule}:{int(rule.stop)}"
     if use_cache:
         hit = _lookup(key)
         if hit:
             return api.make_launch(hit[0], hit[2], hit[1]), [tuple(hit) + ("cached",)]
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
-    cands = candidates(p, n, cus)
+    cands = candidates(p, n, cus, stop=rule.stop)
     if max_candidates:
         cands = cands[:max_candidates]
     buf = api.seed_rows(seeds, rows)

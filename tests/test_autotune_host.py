@@ -30,10 +30,17 @@ def test_shipped_shapes_are_candidates_of_their_problem():
         kind, arch, cus, *rest = key.split(":")
         assert arch == "gfx950" and cus == "256"
         if kind == "plain":
-            p, n = int(rest[0]), int(rest[1])
-            cands = autotune.candidates(p, n, 256)
+            p, n, stop = int(rest[0]), int(rest[1]), rest[3] == "1"
+            cands = autotune.candidates(p, n, 256, stop=stop)
         else:
             p, n = int(rest[1]), int(rest[2])
             cands = autotune.candidates(p, n, 256, batch=True)
             cands += [(s, b, min(t, int(math.ceil(p / 256) * 256))) for s, b, t in cands]
         assert (S, block, tile) in cands, key
+
+
+def test_stop_candidates_include_wide_groups_and_small_blocks():
+    full = autotune.candidates(16384, 65536, 256)
+    stop = autotune.candidates(16384, 65536, 256, stop=True)
+    assert max(c[0] for c in full) <= 8  # enough waves already at S = 4
+    assert (32, 64, 1024) in stop and min(c[1] for c in stop) == 64
