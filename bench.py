@@ -53,6 +53,10 @@ def parse():
                     help="lines of the workload timed on the CPU (multiple of 64)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extras", default="all",
+                    help="comma-separated extra workloads to run (default: all): config1_with_line_ends, "
+                         "config2_batched, config3_adjoint, config4_inverse_fit, fp64_kernel, "
+                         "legacy_default_preset, particle_step")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -203,7 +207,8 @@ def main():
         out["roofline"]["traffic_source"] = "profiles/trace_config1_hbm_traffic.md (rocprofv3 " \
             "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, scales from a 1 GiB copy)"
     if not args.no_extras:
-        out["extras"] = extras(w, charges, seeds, value, launch)
+        only = None if args.extras == "all" else set(args.extras.split(","))
+        out["extras"] = extras(w, charges, seeds, value, launch, only)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(w, args.cpu_lines)
     print(json.dumps(out), flush=True)
 
@@ -226,145 +231,156 @@ def profiled_traffic(workload, shape):
                           shape)
 
 
-def extras(w, charges, seeds, plain_value, launch):
+def extras(w, charges, seeds, plain_value, launch, only=None):
+    """The extra workloads (all, or those named in ``only``); returns their records."""
     import torch
     import emtrace
     from emtrace import autotune, gen, rules
     res = {}
+
+    def on(name):
+        return only is None or name in only
+
     # 1. same workload, lines end where they reverse (useful interactions only)
-    rule_s = rules.angle(stop=True)
-    launch_s, rep_s = autotune.tune_plain(charges, seeds, rule_s)
-    rows = emtrace.seed_rows(seeds, w.len)
-    st = torch.empty(w.n, dtype=torch.int32, device="cuda")
-    wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule_s, launch_s, steps=st), 1, 1)
-    taken = float(st.double().sum().item())
-    res["config1_with_line_ends"] = {
-        "useful_interactions_per_s": w.p * taken / kern[0], "kernel_ms": kern[0] * 1e3,
-        "mean_steps": taken / w.n,
-        "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
-                   "block": launch_s.block},
-        "note": "lines end at sinks; a wave (a block, when the charges are re-staged) stops "
-                "evaluating once all its lines have ended, so the tuned shape has few lines per "
-                "wave and one-wave blocks"}
-    del rows
+    if on("config1_with_line_ends"):
+        rule_s = rules.angle(stop=True)
+        launch_s, rep_s = autotune.tune_plain(charges, seeds, rule_s)
+        rows = emtrace.seed_rows(seeds, w.len)
+        st = torch.empty(w.n, dtype=torch.int32, device="cuda")
+        wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule_s, launch_s, steps=st), 1, 1)
+        taken = float(st.double().sum().item())
+        res["config1_with_line_ends"] = {
+            "useful_interactions_per_s": w.p * taken / kern[0], "kernel_ms": kern[0] * 1e3,
+            "mean_steps": taken / w.n,
+            "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
+                       "block": launch_s.block},
+            "note": "lines end at sinks; a wave (a block, when the charges are re-staged) stops "
+                    "evaluating once all its lines have ended, so the tuned shape has few lines per "
+                    "wave and one-wave blocks"}
+        del rows
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
-    ws = gen.config2()
-    rule_b = rules.angle(stop=False)
-    blaunch, _ = autotune.tune_batch(ws, rule_b)
-    layout, bch, blines = emtrace.pack_batch(ws)
-    plan = emtrace.BatchPlan(layout, rule_b, blaunch, use_graph=True)
-    wall, kern = timed(lambda: plan.run(bch, blines), 2, 1)
-    binter = float(layout.interactions())
-    bval = binter * 2 / wall
-    info = plan.info()
-    res["config2_batched"] = {
-        "interactions": binter, "interactions_per_s": bval, "kernel_ms": min(kern) * 1e3,
-        "ratio_to_plain": bval / plain_value, "configs": len(ws), "items": info["items"],
-        "grid": info["grid"], "launch": {k: info[k] for k in ("lanes_per_line", "tile", "block")}}
-    plan.close()
-    del blines, bch
+    if on("config2_batched"):
+        ws = gen.config2()
+        rule_b = rules.angle(stop=False)
+        blaunch, _ = autotune.tune_batch(ws, rule_b)
+        layout, bch, blines = emtrace.pack_batch(ws)
+        plan = emtrace.BatchPlan(layout, rule_b, blaunch, use_graph=True)
+        wall, kern = timed(lambda: plan.run(bch, blines), 2, 1)
+        binter = float(layout.interactions())
+        bval = binter * 2 / wall
+        info = plan.info()
+        res["config2_batched"] = {
+            "interactions": binter, "interactions_per_s": bval, "kernel_ms": min(kern) * 1e3,
+            "ratio_to_plain": bval / plain_value, "configs": len(ws), "items": info["items"],
+            "grid": info["grid"], "launch": {k: info[k] for k in ("lanes_per_line", "tile", "block")}}
+        plan.close()
+        del blines, bch
     # 3. differentiation workload: forward vs forward + adjoint
-    w3 = gen.config3()
-    tq = gen.perturbed(w3.charges, 0.05)
-    c3 = torch.from_numpy(w3.charges).cuda()
-    s3 = torch.from_numpy(w3.seeds).cuda()
-    rule_d = rules.angle()
-    tgt = emtrace.trace(torch.from_numpy(tq).cuda(), s3, w3.len, rule_d)
-    every = 50
-    rows3 = emtrace.seed_rows(s3, w3.len)
-    ck = torch.empty(emtrace.ckpt_size(w3.n, w3.len, every), dtype=torch.float32, device="cuda")
+    if on("config3_adjoint"):
+        w3 = gen.config3()
+        tq = gen.perturbed(w3.charges, 0.05)
+        c3 = torch.from_numpy(w3.charges).cuda()
+        s3 = torch.from_numpy(w3.seeds).cuda()
+        rule_d = rules.angle()
+        tgt = emtrace.trace(torch.from_numpy(tq).cuda(), s3, w3.len, rule_d)
+        every = 50
+        rows3 = emtrace.seed_rows(s3, w3.len)
+        ck = torch.empty(emtrace.ckpt_size(w3.n, w3.len, every), dtype=torch.float32, device="cuda")
 
-    def fwd():
-        emtrace.trace_into(c3, rows3, w3.n, rule_d, None, ck, every)
+        def fwd():
+            emtrace.trace_into(c3, rows3, w3.n, rule_d, None, ck, every)
 
-    def fwd_adj():
-        fwd()
-        g = (2.0 / (w3.n * w3.len)) * (rows3 - tgt)
-        emtrace.adjoint(c3, rows3, ck, every, g, rule_d)
+        def fwd_adj():
+            fwd()
+            g = (2.0 / (w3.n * w3.len)) * (rows3 - tgt)
+            emtrace.adjoint(c3, rows3, ck, every, g, rule_d)
 
-    _, kf = timed(fwd, 5, 2)
-    _, kfa = timed(fwd_adj, 5, 2)
-    res["config3_adjoint"] = {"forward_ms": min(kf) * 1e3, "forward_adjoint_ms": min(kfa) * 1e3,
-                              "ratio": min(kfa) / min(kf), "ckpt_every": every}
-    del rows3, ck, tgt
+        _, kf = timed(fwd, 5, 2)
+        _, kfa = timed(fwd_adj, 5, 2)
+        res["config3_adjoint"] = {"forward_ms": min(kf) * 1e3, "forward_adjoint_ms": min(kfa) * 1e3,
+                                  "ratio": min(kfa) / min(kf), "ckpt_every": every}
+        del rows3, ck, tgt
     # 4. inverse fit: 64 configurations, 800 rows, 200 Adam iterations, each one
     #    captured batched forward + adjoint + update
-    from emtrace.fit import InverseFit
-    ws4 = gen.config4()
-    truth = [gen.perturbed(w4.charges, 0.1, seed=i) for i, w4 in enumerate(ws4)]
-    fit = InverseFit(ws4, truth, rules.angle(), ckpt_every=50)
-    fit.plan.run(fit.theta, fit.lines)
-    loss0 = fit.loss()
-    fit.step()  # eager warm-up iteration + graph capture
-    torch.cuda.synchronize()
-    iters = 199
-    t0 = time.perf_counter()
-    for _ in range(iters):
-        fit.step()
-    torch.cuda.synchronize()
-    per_iter = (time.perf_counter() - t0) / iters
-    fit.plan.run(fit.theta, fit.lines)
-    res["config4_inverse_fit"] = {
-        "configs": len(ws4), "charges": int(fit.layout.p.sum()), "lines": int(fit.layout.n.sum()),
-        "iterations": iters + 1, "ms_per_iteration": per_iter * 1e3, "loss_start": loss0,
-        "loss_end": fit.loss(), "recovery": fit.recovery(),
-        "note": "iteration = batched forward with checkpoints + dL/dx + batched adjoint + Adam, "
-                "replayed from one captured graph"}
-    del fit
+    if on("config4_inverse_fit"):
+        from emtrace.fit import InverseFit
+        ws4 = gen.config4()
+        truth = [gen.perturbed(w4.charges, 0.1, seed=i) for i, w4 in enumerate(ws4)]
+        fit = InverseFit(ws4, truth, rules.angle(), ckpt_every=50)
+        fit.plan.run(fit.theta, fit.lines)
+        loss0 = fit.loss()
+        fit.step()  # eager warm-up iteration + graph capture
+        torch.cuda.synchronize()
+        iters = 199
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fit.step()
+        torch.cuda.synchronize()
+        per_iter = (time.perf_counter() - t0) / iters
+        fit.plan.run(fit.theta, fit.lines)
+        res["config4_inverse_fit"] = {
+            "configs": len(ws4), "charges": int(fit.layout.p.sum()), "lines": int(fit.layout.n.sum()),
+            "iterations": iters + 1, "ms_per_iteration": per_iter * 1e3, "loss_start": loss0,
+            "loss_end": fit.loss(), "recovery": fit.recovery(),
+            "note": "iteration = batched forward with checkpoints + dL/dx + batched adjoint + Adam, "
+                    "replayed from one captured graph"}
+        del fit
     # 5. FP64 kernel (IEEE binary64, reference op order): config-1 charges and
     #    lines, 250 rows, plain step
-    rows_n = 250
-    q64, s64 = charges.double(), seeds.double()
-    rule_p = rules.plain(10.0)
-    r64 = torch.empty((3, rows_n, w.n), dtype=torch.float64, device="cuda")
-    r64[:, 0, :] = s64.t()
+    if on("fp64_kernel"):
+        rows_n = 250
+        q64, s64 = charges.double(), seeds.double()
+        rule_p = rules.plain(10.0)
+        r64 = torch.empty((3, rows_n, w.n), dtype=torch.float64, device="cuda")
+        r64[:, 0, :] = s64.t()
 
-    def fp64():
-        emtrace._lib.call("em_trace_dev_f64", q64.data_ptr(), w.p, r64[0].data_ptr(),
-                          r64[1].data_ptr(), r64[2].data_ptr(), w.n, rows_n, w.n,
-                          emtrace.api.ctypes.byref(emtrace.make_step(rule_p)), None,
-                          emtrace.api._stream_handle())
+        def fp64():
+            emtrace._lib.call("em_trace_dev_f64", q64.data_ptr(), w.p, r64[0].data_ptr(),
+                              r64[1].data_ptr(), r64[2].data_ptr(), w.n, rows_n, w.n,
+                              emtrace.api.ctypes.byref(emtrace.make_step(rule_p)), None,
+                              emtrace.api._stream_handle())
 
-    _, k64 = timed(fp64, 2, 1)
-    i64 = float(w.p) * w.n * (rows_n - 1)
-    res["fp64_kernel"] = {"interactions_per_s": i64 / min(k64), "kernel_ms": min(k64) * 1e3,
-                          "rows": rows_n, "step_rule": "plain",
-                          "frac_fp64_peak": i64 / min(k64) * FLOP_PER_INTERACTION / 78.6e12,
-                          "note": "one lane per line, charges summed in index order; correctly "
-                                  "rounded sqrt and 3 divisions per interaction"}
-    del r64, q64, s64
+        _, k64 = timed(fp64, 2, 1)
+        i64 = float(w.p) * w.n * (rows_n - 1)
+        res["fp64_kernel"] = {"interactions_per_s": i64 / min(k64), "kernel_ms": min(k64) * 1e3,
+                              "rows": rows_n, "step_rule": "plain",
+                              "frac_fp64_peak": i64 / min(k64) * FLOP_PER_INTERACTION / 78.6e12,
+                              "note": "one lane per line, charges summed in index order; correctly "
+                                      "rounded sqrt and 3 divisions per interaction"}
+        del r64, q64, s64
     # 6. the reference application's default problem (ElectroMag.cpp:43: 1,024
     #    charges, 128 x 128 grid seeds, 2,500 rows, curvature step, resolution 1)
-    wl = emtrace.legacy.preset("default")
-    cl, sl = torch.from_numpy(wl.charges).cuda(), torch.from_numpy(wl.seeds).cuda()
-    rl = emtrace.seed_rows(sl, wl.len)
-    _, kl = timed(lambda: emtrace.trace_into(cl, rl, wl.n, rules.ref(1.0)), 3, 1)
-    res["legacy_default_preset"] = {
-        "interactions_per_s": float(wl.interactions) / min(kl), "kernel_ms": min(kl) * 1e3,
-        "charges": wl.p, "lines": wl.n, "rows": wl.len}
-    del rl, cl, sl
+    if on("legacy_default_preset"):
+        wl = emtrace.legacy.preset("default")
+        cl, sl = torch.from_numpy(wl.charges).cuda(), torch.from_numpy(wl.seeds).cuda()
+        rl = emtrace.seed_rows(sl, wl.len)
+        _, kl = timed(lambda: emtrace.trace_into(cl, rl, wl.n, rules.ref(1.0)), 3, 1)
+        res["legacy_default_preset"] = {
+            "interactions_per_s": float(wl.interactions) / min(kl), "kernel_ms": min(kl) * 1e3,
+            "charges": wl.p, "lines": wl.n, "rows": wl.len}
+        del rl, cl, sl
     # 7. particle step (Particle_System.cpp:108-190): 65,536 particles among the
     #    16,384 config-1 charges, one step with the written-back state
-    import numpy as np
-    rng = np.random.default_rng(0)
-    npart = 65536
-    P = np.zeros((npart, 8), np.float32)
-    P[:, :3] = rng.uniform(-50, 50, (npart, 3))
-    P[:, 3] = rng.uniform(-1e-3, 1e-3, npart)
-    P[:, 4:7] = rng.normal(0, 1, (npart, 3))
-    P[:, 7] = 1.0
-    dP = torch.from_numpy(P).cuda()
-    dA = torch.zeros((npart, 3), dtype=torch.float32, device="cuda")
-    dO = torch.empty_like(dP)
-    _, kv = timed(lambda: emtrace.verlet_step_dev(dP, charges, dA, [-50] * 3, [50] * 3, 1e-3, out=dO),
-                  3, 1)
-    pairs = float(npart) * (w.p + npart - 1)
-    res["particle_step"] = {"particles": npart, "static_charges": w.p, "interactions": pairs,
-                            "interactions_per_s": pairs / min(kv), "kernel_ms": min(kv) * 1e3,
-                            "note": "static terms 12 VALU + rsq, particle-particle terms (Coulomb + "
-                                    "Biot-Savart, self masked) 22 VALU + rsq"}
+    if on("particle_step"):
+        import numpy as np
+        rng = np.random.default_rng(0)
+        npart = 65536
+        P = np.zeros((npart, 8), np.float32)
+        P[:, :3] = rng.uniform(-50, 50, (npart, 3))
+        P[:, 3] = rng.uniform(-1e-3, 1e-3, npart)
+        P[:, 4:7] = rng.normal(0, 1, (npart, 3))
+        P[:, 7] = 1.0
+        dP = torch.from_numpy(P).cuda()
+        dA = torch.zeros((npart, 3), dtype=torch.float32, device="cuda")
+        dO = torch.empty_like(dP)
+        _, kv = timed(lambda: emtrace.verlet_step_dev(dP, charges, dA, [-50] * 3, [50] * 3, 1e-3, out=dO),
+                      3, 1)
+        pairs = float(npart) * (w.p + npart - 1)
+        res["particle_step"] = {"particles": npart, "static_charges": w.p, "interactions": pairs,
+                                "interactions_per_s": pairs / min(kv), "kernel_ms": min(kv) * 1e3,
+                                "note": "static terms 12 VALU + rsq, particle-particle terms (Coulomb + "
+                                        "Biot-Savart, self masked) 22 VALU + rsq"}
     return res
-
 
 if __name__ == "__main__":
     main()

@@ -504,8 +504,24 @@ k_adjoint_reduce(const Cfg* __restrict__ cfgs, const RedItem* __restrict__ items
     const uint32_t ci = it.charge0 + threadIdx.x;
     if (ci >= cfg.p) return;
     float sx = 0.f, sy = 0.f, sz = 0.f, sq = 0.f, ex = 0.f, ey = 0.f, ez = 0.f, eq = 0.f;
-    for (uint32_t c = 0; c < it.chunks; ++c) {
-        const float4 v = part[it.part + (uint64_t)c * cfg.p + threadIdx.x];
+    const float4* src = part + it.part + threadIdx.x;
+    // chunk partials in index order; loads issued eight at a time
+    constexpr uint32_t kBatch = 8;
+    uint32_t c = 0;
+    for (; c + kBatch <= it.chunks; c += kBatch) {
+        float4 v[kBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) v[b] = src[(uint64_t)(c + b) * cfg.p];
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            fold(sx, ex, v[b].x);
+            fold(sy, ey, v[b].y);
+            fold(sz, ez, v[b].z);
+            fold(sq, eq, v[b].w);
+        }
+    }
+    for (; c < it.chunks; ++c) {
+        const float4 v = src[(uint64_t)c * cfg.p];
         fold(sx, ex, v.x);
         fold(sy, ey, v.y);
         fold(sz, ez, v.z);
