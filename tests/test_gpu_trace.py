@@ -216,3 +216,52 @@ def test_drop_in_matches_device_path(gpu):
     dev_rows = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.ref(10.0))
     assert np.array_equal(rows, dev_rows)
     assert perf["h2d_bytes"] > 0 and perf["d2h_bytes"] == 3 * 4 * w.n * (w.len - 1)
+
+
+EDGE = [
+    # (name, p, n, len, rule, launch (S, tile, block) or None)
+    ("one_line", 50, 1, 200, rules.plain(10.0), None),
+    ("two_rows", 300, 130, 2, rules.angle(), None),
+    ("ragged_65_lines", 100, 65, 120, rules.ref(10.0), None),
+    ("p_equals_tile", 256, 96, 80, rules.plain(10.0), (2, 256, 256)),
+    ("p_tile_plus_one", 257, 96, 80, rules.plain(10.0), (2, 256, 256)),
+    ("p_tile_minus_one", 255, 96, 80, rules.angle(), (2, 256, 256)),
+    ("wide_groups_tiny_p", 40, 64, 60, rules.angle(), (32, 512, 64)),
+]
+
+
+@pytest.mark.parametrize("case", EDGE, ids=[c[0] for c in EDGE])
+def test_edge_shapes_vs_cpu_path(gpu, orc, case):
+    """Shapes at the edges of the work split and of the charge tiles, against
+    the CPU path: same step counts, points within 1e-4 before the first
+    approach within 0.05 of a charge (REF needs n % 64 on the CPU path: lines
+    are padded there and cut)."""
+    from helpers import line_errors, prefix_lengths
+    name, p, n, length, rule, shape = case
+    w = gen.plain_config(p, n, 5.0, length, seed=21)
+    launch = gpu.make_launch(shape[0], shape[1], shape[2]) if shape else None
+    rows, steps = gpu.trace(dev(w.charges), dev(w.seeds), length, rule, launch, with_steps=True)
+    rows = rows.cpu().numpy()
+    seeds = w.seeds
+    if rule.rule == rules.REF and n % 64:
+        seeds = np.concatenate([seeds, np.repeat(seeds[:1], 64 - n % 64, axis=0)])
+    rc, c_rows, c_steps = orc.calc_lines(w.charges, seeds, length, rule, with_steps=True)
+    assert rc == 0
+    c_rows, c_steps = c_rows[:, :, :n], c_steps[:n]
+    assert np.array_equal(steps.cpu().numpy().astype(np.int64), c_steps.astype(np.int64))
+    first, _ = prefix_lengths(c_rows, w.charges)
+    err = line_errors(rows, c_rows, first)
+    print(f"[edge {name}] lines {n}: median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert np.isfinite(rows).all()
+    assert (err <= 1e-4).mean() >= 0.95 and np.median(err) <= 1e-5
+
+
+def test_no_charges_gives_the_reference_nan(gpu, orc):
+    """p = 0: E = 0 and the plain step divides 0 by 0, as the reference does."""
+    w = gen.plain_config(4, 64, 5.0, 6, seed=2)
+    empty = np.zeros((0, 4), np.float32)
+    rows = gpu.trace(dev(empty), dev(w.seeds), 6, rules.plain(10.0)).cpu().numpy()
+    rc, c_rows = orc.calc_lines(empty, w.seeds, 6, rules.plain(10.0))
+    assert rc == 0
+    assert np.array_equal(rows[:, 0], c_rows[:, 0])
+    assert np.isnan(rows[:, 1:]).all() and np.isnan(c_rows[:, 1:]).all()
