@@ -5,7 +5,8 @@
 //   (ElectroMag/src/CPU Implement.h:25-30, called at ElectroMag.cpp:305-306)
 // and the GPU stub CalcField(...) (GPGPU_Segment/src/Electrostatics.cpp:28-42)
 // with emtrace::CalcField_GPU, which has the same arguments and return codes
-// and runs the gfx950 kernels through the C ABI in emtrace.h.
+// and runs the gfx950 kernels through the C ABI in emtrace.h; emtrace::CalcField
+// takes the stub's array-of-structures lines.
 //
 // CalcField_GPU is generic over the containers: it only uses the public
 // interface the reference's types expose --
@@ -27,6 +28,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../emtrace.h"
@@ -50,6 +52,20 @@ inline int calc_lines(double* x, double* y, double* z, const void* q, size_t p, 
 }
 }  // namespace detail
 
+namespace detail {
+template <class Perf>
+void fill_perf(Perf& perfData, const em_perf& p)
+{
+    typedef typename decltype(perfData.stepTimes)::value_type Timing;
+    perfData.time = p.time_s;
+    perfData.performance = p.gflops;
+    perfData.add(Timing("Host to device transfer", p.h2d_s, (size_t)p.h2d_bytes));
+    perfData.add(Timing("Kernel execution time", p.kernel_s));
+    perfData.add(Timing("Device to host transfer", p.d2h_s, (size_t)p.d2h_bytes));
+    perfData.progress = 1.0;
+}
+}  // namespace detail
+
 template <class LinesSoA, class ChargeArray, class Perf>
 int CalcField_GPU(LinesSoA& fieldLines, ChargeArray& pointCharges, const size_t n,
                   double resolution, Perf& perfData, bool useCurvature = false)
@@ -62,13 +78,42 @@ int CalcField_GPU(LinesSoA& fieldLines, ChargeArray& pointCharges, const size_t 
     const int rc = detail::calc_lines(lines.x, lines.y, lines.z, pointCharges.GetDataPointer(),
                                       pointCharges.GetSize(), n, len, resolution, useCurvature, &p);
     if (rc != EM_OK) return rc;
-    typedef typename decltype(perfData.stepTimes)::value_type Timing;
-    perfData.time = p.time_s;
-    perfData.performance = p.gflops;
-    perfData.add(Timing("Host to device transfer", p.h2d_s, (size_t)p.h2d_bytes));
-    perfData.add(Timing("Kernel execution time", p.kernel_s));
-    perfData.add(Timing("Device to host transfer", p.d2h_s, (size_t)p.d2h_bytes));
-    perfData.progress = 1.0;
+    detail::fill_perf(perfData, p);
+    return rc;
+}
+
+// The reference's GPU entry point with array-of-structures lines,
+//   int CalcField(Array<Vector3<float>>& fieldLines, Array<pointCharge<float>>&,
+//                 size_t n, float resolution, perfPacket&, bool useCurvature)
+// (GPGPU_Segment/src/Electrostatics.cpp:28-42, a stub returning -1 there; the
+// double overload likewise).  Element (line, step) is fieldLines[step*n + line]
+// with x, y, z members; row 0 holds the seeds.  The rows go through planar
+// buffers to the same kernels as CalcField_GPU.
+template <class LinesAoS, class ChargeArray, class Perf>
+int CalcField(LinesAoS& fieldLines, ChargeArray& pointCharges, const size_t n, double resolution,
+              Perf& perfData, bool useCurvature = false)
+{
+    typedef typename std::remove_reference<decltype(fieldLines[0].x)>::type T;
+    static_assert(sizeof(*pointCharges.GetDataPointer()) == 4 * sizeof(T),
+                  "point charges must be {x, y, z, q} records of the line element type");
+    const size_t total = fieldLines.GetSize();
+    const size_t len = n ? total / n : 0;
+    std::vector<T> x(n * len), y(n * len), z(n * len);
+    for (size_t i = 0; i < n && i < total; ++i) {
+        x[i] = fieldLines[i].x;
+        y[i] = fieldLines[i].y;
+        z[i] = fieldLines[i].z;
+    }
+    em_perf p = {};
+    const int rc = detail::calc_lines(x.data(), y.data(), z.data(), pointCharges.GetDataPointer(),
+                                      pointCharges.GetSize(), n, len, resolution, useCurvature, &p);
+    if (rc != EM_OK) return rc;
+    for (size_t i = n; i < n * len; ++i) {
+        fieldLines[i].x = x[i];
+        fieldLines[i].y = y[i];
+        fieldLines[i].z = z[i];
+    }
+    detail::fill_perf(perfData, p);
     return rc;
 }
 

@@ -1,7 +1,9 @@
 // Drop-in check: the reference's call site shape (ElectroMag.cpp:236-306) with
 // emtrace::CalcField_GPU in place of CalcField_CPU<float>.  Reads charges and
 // seeds from a binary file, traces, writes the rows and the perf record.
-//   dropin_main <in.bin> <out.bin> <resolution> <use_curvature>
+//   dropin_main <in.bin> <out.bin> <resolution> <use_curvature> [aos]
+// With "aos" the lines are an Array<Vector3<float>> and the call is
+// emtrace::CalcField (the array-of-structures entry point).
 // in.bin : uint64 p, uint64 n, uint64 len, p*4 float charges, n*3 float seeds
 // out.bin: int32 status, double time, double performance, double progress,
 //          uint32 steps, then 3 * len * n floats (x rows, y rows, z rows)
@@ -16,7 +18,8 @@ using namespace emtrace::compat;
 
 int main(int argc, char** argv)
 {
-    if (argc != 5) return 64;
+    if (argc != 5 && argc != 6) return 64;
+    const bool aos = argc == 6;
     FILE* f = std::fopen(argv[1], "rb");
     if (!f) return 65;
     uint64_t hdr[3];
@@ -31,8 +34,17 @@ int main(int argc, char** argv)
     std::fclose(f);
     for (size_t i = 0; i < n; ++i) lines.write({seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]}, i);
     perfPacket perf;
-    const int rc = emtrace::CalcField_GPU(lines, charges, n, std::atof(argv[3]), perf,
-                                          std::atoi(argv[4]) != 0);
+    int rc;
+    if (aos) {
+        Array<Vector3<float> > al;
+        if (al.AlignAlloc(n * len)) return 71;
+        for (size_t i = 0; i < n * len; ++i) al[i] = lines[i];
+        rc = emtrace::CalcField(al, charges, n, std::atof(argv[3]), perf, std::atoi(argv[4]) != 0);
+        for (size_t i = 0; i < n * len; ++i) lines.write(al[i], i);
+    } else {
+        rc = emtrace::CalcField_GPU(lines, charges, n, std::atof(argv[3]), perf,
+                                    std::atoi(argv[4]) != 0);
+    }
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 70;
     int32_t st = rc;

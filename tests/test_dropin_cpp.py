@@ -33,6 +33,22 @@ int run(size_t n, size_t len, size_t p) {
     perfPacket CPUperf;
     return emtrace::CalcField_GPU(CPUlines, charges, n, 1.0f, CPUperf, true);
 }
+int run_aos(size_t n, size_t len, size_t p) {  // the GPU stub's shape (Electrostatics.cpp:28-42)
+    Array<Vector3<float> > lines;
+    Array<electro::pointCharge<float> > charges;
+    lines.AlignAlloc(n * len);
+    charges.AlignAlloc(p);
+    perfPacket perf;
+    return emtrace::CalcField(lines, charges, n, 1.0f, perf, true);
+}
+int run_f64(size_t n, size_t len, size_t p) {  // CalcField_CPU<double>'s containers
+    Vector3<Array<double> > lines;
+    Array<electro::pointCharge<double> > charges;
+    lines.AlignAlloc(n * len);
+    charges.AlignAlloc(p);
+    perfPacket perf;
+    return emtrace::CalcField_GPU(lines, charges, n, 1.0, perf, true);
+}
 '''
 
 
@@ -60,7 +76,7 @@ def test_header_and_program_compile_standalone():
                        check=True)
 
 
-def _run_dropin(charges, seeds, length, resolution, curvature):
+def _run_dropin(charges, seeds, length, resolution, curvature, aos=False):
     exe = build_dropin()
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
@@ -68,7 +84,8 @@ def _run_dropin(charges, seeds, length, resolution, curvature):
             f.write(struct.pack("<QQQ", charges.shape[0], seeds.shape[0], length))
             f.write(np.ascontiguousarray(charges, np.float32).tobytes())
             f.write(np.ascontiguousarray(seeds, np.float32).tobytes())
-        subprocess.run([exe, fin, fout, str(resolution), str(int(curvature))], check=True, timeout=120)
+        subprocess.run([exe, fin, fout, str(resolution), str(int(curvature))] + (["aos"] if aos else []),
+                       check=True, timeout=120)
         raw = open(fout, "rb").read()
     st, t, g, pr, steps = struct.unpack("<idddI", raw[:32])
     rows = np.frombuffer(raw[32:], np.float32).reshape(3, length, seeds.shape[0])
@@ -85,6 +102,17 @@ def test_dropin_program_matches_device_path(gpu):
     dev_rows = gpu.trace(torch.from_numpy(w.charges).cuda(), torch.from_numpy(w.seeds).cuda(),
                          w.len, rules.ref(10.0)).cpu().numpy()
     assert np.array_equal(rows, dev_rows)
+
+
+@pytest.mark.gpu
+def test_aos_entry_point_matches_soa(gpu):
+    """emtrace::CalcField on Array<Vector3<float>> lines writes the same rows."""
+    from emtrace import gen
+    w = gen.plain_config(300, 192, 10.0, 90, seed=17)
+    soa = _run_dropin(w.charges, w.seeds, w.len, 10.0, True)
+    aos = _run_dropin(w.charges, w.seeds, w.len, 10.0, True, aos=True)
+    assert soa[0] == 0 and aos[0] == 0 and aos[3] == 1.0 and aos[4] == 3
+    assert np.array_equal(soa[5], aos[5])
 
 
 @pytest.mark.gpu
