@@ -80,10 +80,13 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
             }
         }
         if (writer) {
+            // ordinary (L2 write-back) stores: with S lanes per line a wave
+            // writes 64/S consecutive lines, part of a 128-byte line per row;
+            // the neighbouring waves' parts meet in L2 before they reach HBM
             const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + line;
-            __builtin_nontemporal_store(st.px, X + o);
-            __builtin_nontemporal_store(st.py, Y + o);
-            __builtin_nontemporal_store(st.pz, Z + o);
+            X[o] = st.px;
+            Y[o] = st.py;
+            Z[o] = st.pz;
             if (ckpt && s % ckpt_every == 0) store_ckpt(ckpt, cfg.ckpt_off, s / ckpt_every, cfg.n, line, st);
         }
     }
