@@ -113,8 +113,8 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
     }
     if (req && req->tile) {
         s.tile = (uint32_t)req->tile;
-    } else if (stop && restaged && (uint32_t)(s.S * kGroup) <= 1024u) {
-        s.tile = 1024;
+    } else if (stop && restaged) {
+        s.tile = std::max<uint32_t>(512u, (uint32_t)(s.S * kGroup));  // many one-wave blocks per CU
     } else {
         const uint64_t blocks = (n * (uint64_t)s.S + s.block - 1) / s.block;
         const int resident = (int)std::min<uint64_t>(std::max<uint64_t>(1, (blocks + dev.cus - 1) / dev.cus),

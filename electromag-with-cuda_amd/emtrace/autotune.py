@@ -67,9 +67,9 @@ def _device_key() -> str:
 # 1%) going to fewer lanes per line.  [S, block, tile, ms at 48 rows]
 SHIPPED = {
     # 16,384 charges, 65,536 lines, angle rule (all steps / with line ends:
-    # 239 ms vs 917 ms for the all-steps shape)
+    # 210 ms vs 917 ms for the all-steps shape)
     "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, None],
-    "plain:gfx950:256:16384:65536:2:1": [32, 64, 1024, None],
+    "plain:gfx950:256:16384:65536:2:1": [32, 64, 512, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule
     "batch:gfx950:256:512:8192:1048576:2:0": [4, 256, 4096, None],
 }
@@ -123,7 +123,7 @@ def candidates(p: int, n: int, cus: int, batch: bool = False,
             need = _round_up(max(p, 1), q)
             tiles = set()
             if stop and not batch:
-                tiles.update(t for t in (1024, 2048, 4096) if t % q == 0 and t < need)
+                tiles.update(t for t in (512, 1024, 2048, 4096) if t % q == 0 and t < need)
             for resident in (1, 2):
                 tmax = (LDS_BYTES // resident // 16) // q * q
                 if tmax < q:

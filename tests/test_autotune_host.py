@@ -43,4 +43,4 @@ def test_stop_candidates_include_wide_groups_and_small_blocks():
     full = autotune.candidates(16384, 65536, 256)
     stop = autotune.candidates(16384, 65536, 256, stop=True)
     assert max(c[0] for c in full) <= 8  # enough waves already at S = 4
-    assert (32, 64, 1024) in stop and min(c[1] for c in stop) == 64
+    assert (32, 64, 512) in stop and min(c[1] for c in stop) == 64
