@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=["config1", "config0"])
+    ap.add_argument("--workload", default="lines_64k", choices=["lines_64k", "lines_8k"])
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
@@ -54,8 +54,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extras", default="all",
-                    help="comma-separated extra workloads to run (default: all): config1_with_line_ends, "
-                         "config2_batched, config3_adjoint, config4_inverse_fit, fp64_kernel, "
+                    help="comma-separated extra workloads to run (default: all): lines_64k_with_line_ends, "
+                         "batch_512, adjoint_2k, fit_64, fp64_kernel, "
                          "legacy_default_preset, particle_step")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH/WRITE_SIZE counters)")
@@ -139,7 +139,7 @@ def main():
         return copy_only()
 
     rule = rules.angle(stop=False)
-    w = gen.config1() if args.workload == "config1" else gen.config0()
+    w = gen.lines_64k() if args.workload == "lines_64k" else gen.lines_8k()
     charges = torch.from_numpy(w.charges).cuda()
     seeds = torch.from_numpy(w.seeds).cuda()
     if args.lanes or args.tile or args.block:
@@ -177,7 +177,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: plain method, {w.p} charges in "
-                               f"[-{50 if args.workload == 'config1' else 10}, ...]^3, {w.n} lines, "
+                               f"[-{50 if args.workload == 'lines_64k' else 10}, ...]^3, {w.n} lines, "
                                f"{w.len} rows, curvature-adaptive (angle) step in [0.005, 0.5], "
                                f"every line all {w.len - 1} steps",
                    "charges": w.p, "lines": w.n, "rows": w.len, "step_rule": "angle",
@@ -193,18 +193,18 @@ def main():
                      "kernel_ms": k_avg * 1e3, "flop_per_launch": inter * FLOP_PER_INTERACTION,
                      "algorithmic_bytes_per_launch": 12.0 * w.n * (w.len - 1)},
     }
-    hw_flop = profiled_value("trace_config1_valu_counters.md", "fp32_flop_per_launch",
+    hw_flop = profiled_value("trace_lines64k_valu_counters.md", "fp32_flop_per_launch",
                              args.workload, shape)
     if hw_flop is not None:
         out["roofline"]["rocprof_fp32"] = {
             "flop_per_launch": hw_flop, "achieved": hw_flop / k_avg / 1e12,
             "frac": hw_flop / k_avg / 1e12 / PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "source": "profiles/trace_config1_valu_counters.md (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 "
+            "source": "profiles/trace_lines64k_valu_counters.md (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 "
                       "x 64 per dispatch) over this run's kernel time"}
     traffic = profiled_traffic(args.workload, shape)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
-        out["roofline"]["traffic_source"] = "profiles/trace_config1_hbm_traffic.md (rocprofv3 " \
+        out["roofline"]["traffic_source"] = "profiles/trace_lines64k_hbm_traffic.md (rocprofv3 " \
             "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, scales from a 1 GiB copy)"
     if not args.no_extras:
         only = None if args.extras == "all" else set(args.extras.split(","))
@@ -217,7 +217,7 @@ def profiled_value(name, key, workload, shape):
     """A per-launch figure from a committed counter profile, when this run used
     the launch shape that was profiled (lanes 4, tile 4096, block 512)."""
     path = os.path.join(ROOT, "profiles", name)
-    if workload != "config1" or shape != (4, 4096, 512) or not os.path.exists(path):
+    if workload != "lines_64k" or shape != (4, 4096, 512) or not os.path.exists(path):
         return None
     for line in open(path):
         if f"{key}:" in line:
@@ -227,7 +227,7 @@ def profiled_value(name, key, workload, shape):
 
 def profiled_traffic(workload, shape):
     """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE)."""
-    return profiled_value("trace_config1_hbm_traffic.md", "traffic_bytes_per_launch", workload,
+    return profiled_value("trace_lines64k_hbm_traffic.md", "traffic_bytes_per_launch", workload,
                           shape)
 
 
@@ -242,14 +242,14 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
         return only is None or name in only
 
     # 1. same workload, lines end where they reverse (useful interactions only)
-    if on("config1_with_line_ends"):
+    if on("lines_64k_with_line_ends"):
         rule_s = rules.angle(stop=True)
         launch_s, rep_s = autotune.tune_plain(charges, seeds, rule_s)
         rows = emtrace.seed_rows(seeds, w.len)
         st = torch.empty(w.n, dtype=torch.int32, device="cuda")
         wall, kern = timed(lambda: emtrace.trace_into(charges, rows, w.n, rule_s, launch_s, steps=st), 1, 1)
         taken = float(st.double().sum().item())
-        res["config1_with_line_ends"] = {
+        res["lines_64k_with_line_ends"] = {
             "useful_interactions_per_s": w.p * taken / kern[0], "kernel_ms": kern[0] * 1e3,
             "mean_steps": taken / w.n,
             "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
@@ -259,8 +259,8 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                     "wave and one-wave blocks"}
         del rows
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
-    if on("config2_batched"):
-        ws = gen.config2()
+    if on("batch_512"):
+        ws = gen.batch_512()
         rule_b = rules.angle(stop=False)
         blaunch, _ = autotune.tune_batch(ws, rule_b)
         layout, bch, blines = emtrace.pack_batch(ws)
@@ -269,15 +269,15 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
         binter = float(layout.interactions())
         bval = binter * 2 / wall
         info = plan.info()
-        res["config2_batched"] = {
+        res["batch_512"] = {
             "interactions": binter, "interactions_per_s": bval, "kernel_ms": min(kern) * 1e3,
             "ratio_to_plain": bval / plain_value, "configs": len(ws), "items": info["items"],
             "grid": info["grid"], "launch": {k: info[k] for k in ("lanes_per_line", "tile", "block")}}
         plan.close()
         del blines, bch
     # 3. differentiation workload: forward vs forward + adjoint
-    if on("config3_adjoint"):
-        w3 = gen.config3()
+    if on("adjoint_2k"):
+        w3 = gen.adjoint_2k()
         tq = gen.perturbed(w3.charges, 0.05)
         c3 = torch.from_numpy(w3.charges).cuda()
         s3 = torch.from_numpy(w3.seeds).cuda()
@@ -297,14 +297,14 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
 
         _, kf = timed(fwd, 5, 2)
         _, kfa = timed(fwd_adj, 5, 2)
-        res["config3_adjoint"] = {"forward_ms": min(kf) * 1e3, "forward_adjoint_ms": min(kfa) * 1e3,
+        res["adjoint_2k"] = {"forward_ms": min(kf) * 1e3, "forward_adjoint_ms": min(kfa) * 1e3,
                                   "ratio": min(kfa) / min(kf), "ckpt_every": every}
         del rows3, ck, tgt
     # 4. inverse fit: 64 configurations, 800 rows, 200 Adam iterations, each one
     #    captured batched forward + adjoint + update
-    if on("config4_inverse_fit"):
+    if on("fit_64"):
         from emtrace.fit import InverseFit
-        ws4 = gen.config4()
+        ws4 = gen.fit_64()
         truth = [gen.perturbed(w4.charges, 0.1, seed=i) for i, w4 in enumerate(ws4)]
         fit = InverseFit(ws4, truth, rules.angle(), ckpt_every=50)
         fit.plan.run(fit.theta, fit.lines)
@@ -318,14 +318,14 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
         torch.cuda.synchronize()
         per_iter = (time.perf_counter() - t0) / iters
         fit.plan.run(fit.theta, fit.lines)
-        res["config4_inverse_fit"] = {
+        res["fit_64"] = {
             "configs": len(ws4), "charges": int(fit.layout.p.sum()), "lines": int(fit.layout.n.sum()),
             "iterations": iters + 1, "ms_per_iteration": per_iter * 1e3, "loss_start": loss0,
             "loss_end": fit.loss(), "recovery": fit.recovery(),
             "note": "iteration = batched forward with checkpoints + dL/dx + batched adjoint + Adam, "
                     "replayed from one captured graph"}
         del fit
-    # 5. FP64 kernel (IEEE binary64, reference op order): config-1 charges and
+    # 5. FP64 kernel (IEEE binary64, reference op order): lines_64k charges and
     #    lines, 250 rows, plain step
     if on("fp64_kernel"):
         rows_n = 250
@@ -360,7 +360,7 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
             "charges": wl.p, "lines": wl.n, "rows": wl.len}
         del rl, cl, sl
     # 7. particle step (Particle_System.cpp:108-190): 65,536 particles among the
-    #    16,384 config-1 charges, one step with the written-back state
+    #    16,384 lines_64k charges, one step with the written-back state
     if on("particle_step"):
         import numpy as np
         rng = np.random.default_rng(0)

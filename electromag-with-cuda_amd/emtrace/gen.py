@@ -121,17 +121,17 @@ def plain_config(p: int, n: int, half_side: float, length: int, seed: int = 0) -
     return Workload(ch, make_seeds(ch, n, seed=seed), length)
 
 
-def config0(seed: int = 0) -> Workload:
+def lines_8k(seed: int = 0) -> Workload:
     """1,000 charges in [-10,10]^3, 8,192 lines, 2,500 rows."""
     return plain_config(1000, 8192, 10.0, 2500, seed)
 
 
-def config1(seed: int = 0) -> Workload:
+def lines_64k(seed: int = 0) -> Workload:
     """16,384 charges in [-50,50]^3, 65,536 lines, 2,500 rows."""
     return plain_config(16384, 65536, 50.0, 2500, seed)
 
 
-def config3(seed: int = 0) -> Workload:
+def adjoint_2k(seed: int = 0) -> Workload:
     """256 charges in [-5,5]^3, 2,048 lines, 500 rows (differentiation)."""
     return plain_config(256, 2048, 5.0, 500, seed)
 
@@ -154,12 +154,12 @@ def batched_configs(count: int, p_range, n_range, length: int, seed: int = 0):
     return out
 
 
-def config2(seed: int = 0):
+def batch_512(seed: int = 0):
     """512 configurations, p in [64, 8192], n in [256, 4096], 1,000 rows."""
     return batched_configs(512, (64, 8192), (256, 4096), 1000, seed)
 
 
-def config4(seed: int = 0):
+def fit_64(seed: int = 0):
     """64 configurations, p in [128, 1024], n in [512, 2048], 800 rows."""
     return batched_configs(64, (128, 1024), (512, 2048), 800, seed)
 

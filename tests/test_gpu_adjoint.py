@@ -94,8 +94,8 @@ def test_adjoint_multi_tile_matches_single_tile(gpu):
     assert rel(out[1], out[0]) < 1e-3
 
 
-def _config3_problem(seed=0, sigma=0.05):
-    w = gen.config3(seed)
+def _adjoint_2k_problem(seed=0, sigma=0.05):
+    w = gen.adjoint_2k(seed)
     target_q = gen.perturbed(w.charges, sigma, seed=seed)
     return w, target_q
 
@@ -109,11 +109,11 @@ def agreeing_lines(g_rows, g_steps, r64, s64, tol=1e-3):
     return (g_steps == s64) & (err <= tol)
 
 
-def test_config3_gradient_vs_fp64_adjoint_and_fd(gpu, orc):
+def test_adjoint_2k_gradient_vs_fp64_adjoint_and_fd(gpu, orc):
     """256 charges, 2,048 lines, 500 rows, angle rule with stop, checkpoints
     every 50 steps; loss = mean |x - x_target|^2."""
     rule = rules.angle()
-    w, tq = _config3_problem()
+    w, tq = _adjoint_2k_problem()
     npts = w.n * w.len
     tgt = gpu.trace(dev(tq), dev(w.seeds), w.len, rule)
     # full autograd gradient (reported)
@@ -181,7 +181,7 @@ def test_config3_gradient_vs_fp64_adjoint_and_fd(gpu, orc):
     f = fd[1e-4][smooth]
     r_fd64 = float(np.linalg.norm(g64m.ravel()[picks][smooth] - f) / np.linalg.norm(f))
     r_fd = float(np.linalg.norm(g_gpu.ravel()[picks][smooth] - f) / np.linalg.norm(f))
-    print(f"[config3] loss {loss.item():.4e}; agreeing lines {ok.sum()}/{len(ok)}; "
+    print(f"[adjoint_2k] loss {loss.item():.4e}; agreeing lines {ok.sum()}/{len(ok)}; "
           f"mean steps {g_steps.mean():.1f}; rel(gpu, FP64 along gpu path) {r_path:.2e}; "
           f"(a) rel(gpu, fp64 adjoint), same dL/dx: {r_adj:.2e} "
           f"[all lines: {rel(g_full, g64_full):.2e}]; FD(1e-4)-converged params "

@@ -79,9 +79,9 @@ def test_batched_adjoint_equals_single(gpu):
         assert torch.allclose(got, gc, rtol=1e-4, atol=1e-6 * gc.abs().max().item()), i
 
 
-def test_full_size_config1_properties(gpu):
+def test_full_size_lines_64k_properties(gpu):
     """At the headline size: finite output, deterministic, every row written."""
-    w = gen.config1()
+    w = gen.lines_64k()
     c = torch.from_numpy(w.charges).cuda()
     s = torch.from_numpy(w.seeds).cuda()
     rows = gpu.seed_rows(s, w.len)

@@ -2,7 +2,7 @@
 
 * graph replays equal eager iterations bit for bit, with eager work and
   synchronisations between replays;
-* the batched adjoint's gradients on a 4-configuration subset of the config-4
+* the batched adjoint's gradients on a 4-configuration subset of the fit_64
   workload agree with the FP64 oracle: the FP64 sweep along the GPU's own path
   (all lines), and the FP64 adjoint of the FP64 trajectory on the lines that
   follow it;
@@ -46,9 +46,9 @@ def test_graph_replay_equals_eager_iterations(gpu):
         assert torch.equal(graphed.theta, want[k]), k
 
 
-def test_config4_subset_gradients_vs_fp64(gpu, orc):
-    """First 4 configurations of the config-4 workload (800 rows, sigma 0.1)."""
-    ws_all = gen.config4()
+def test_fit_64_subset_gradients_vs_fp64(gpu, orc):
+    """First 4 configurations of the fit_64 workload (800 rows, sigma 0.1)."""
+    ws_all = gen.fit_64()
     ws = ws_all[:4]
     truth = [gen.perturbed(w.charges, 0.1, seed=i) for i, w in enumerate(ws)]
     fit = InverseFit(ws, truth, rules.angle(), ckpt_every=50)
@@ -86,7 +86,7 @@ def test_config4_subset_gradients_vs_fp64(gpu, orc):
         g64, _ = orc.adjoint64(w.charges, seeds, w.len, rules.angle(), Gm)
         Gmf = np.ascontiguousarray(Gm.transpose(2, 0, 1)).astype(np.float32)
         rep.append((w.p, w.n, float(r_path), float(ok.mean()), g64, Gmf, i))
-        print(f"[config4 subset] cfg {i}: p={w.p} n={w.n} rel(gpu, FP64 along gpu path) {r_path:.2e}; "
+        print(f"[fit_64 subset] cfg {i}: p={w.p} n={w.n} rel(gpu, FP64 along gpu path) {r_path:.2e}; "
               f"lines following FP64 {ok.mean():.3f}", flush=True)
         assert r_path < 1e-3
         assert ok.mean() > 0.8
@@ -99,7 +99,7 @@ def test_config4_subset_gradients_vs_fp64(gpu, orc):
     for (p, n, _, _, g64, _, i) in rep:
         o = int(fit.layout.charge_off[i])
         r = np.linalg.norm(grad[o:o + p] - g64) / np.linalg.norm(g64)
-        print(f"[config4 subset] cfg {i}: rel(gpu, FP64 adjoint of FP64 trajectory) {r:.2e}", flush=True)
+        print(f"[fit_64 subset] cfg {i}: rel(gpu, FP64 adjoint of FP64 trajectory) {r:.2e}", flush=True)
         assert r < 1e-1
 
 

@@ -6,7 +6,7 @@
 * against the reference's own double rows (golden vectors) it is as close as
   the restatement is (-ffast-math reassociation in the reference build);
 * the drop-in calc_lines64 keeps the reference's status codes;
-* at the full config-0 size (1,000 charges, 8,192 lines, 2,500 rows) the FP32
+* at the full lines_8k size (1,000 charges, 8,192 lines, 2,500 rows) the FP32
   kernel drifts from the FP64 kernel no more than the CPU FP32 path does;
 * the command line with --cpuprecision=double --gpuprecision=double writes
   identical CPU and GPU text.
@@ -118,14 +118,14 @@ def test_fp64_dropin_status_codes(gpu):
     assert gpu.calc_lines64(ch, x1, y1, z1, 64, 1.0, True)[0] == 3
 
 
-def test_fp32_kernel_vs_fp64_kernel_full_config0(gpu, orc):
-    """Full config 0 on both kernels: the FP64 kernel stands in for the oracle at
+def test_fp32_kernel_vs_fp64_kernel_full_lines_8k(gpu, orc):
+    """All of lines_8k on both kernels: the FP64 kernel stands in for the oracle at
     a size the CPU restatement would take minutes on.  FP32 rows drift from FP64
     over 2,500 steps on any hardware; the GPU's FP32 rows must drift no more
     than the CPU FP32 restatement's (sampled lines, prefixes before the first
     approach within 0.05 of a charge)."""
     from helpers import line_errors, prefix_lengths
-    w = gen.config0()
+    w = gen.lines_8k()
     rule = rules.ref(10.0)
     q32 = torch.from_numpy(w.charges).cuda()
     r32 = gpu.trace(q32, torch.from_numpy(w.seeds).cuda(), w.len, rule).cpu().numpy()
@@ -140,7 +140,7 @@ def test_fp32_kernel_vs_fp64_kernel_full_config0(gpu, orc):
     e_gpu = line_errors(r32[:, :, idx], ref64, first)
     e_cpu = line_errors(c32, ref64, first)
     q = lambda e, f: float(np.quantile(e, f))  # noqa: E731
-    print(f"[config0 vs FP64 kernel, {idx.size} lines] gpu fp32 median {q(e_gpu, .5):.3e} p90 "
+    print(f"[lines_8k vs FP64 kernel, {idx.size} lines] gpu fp32 median {q(e_gpu, .5):.3e} p90 "
           f"{q(e_gpu, .9):.3e} max {e_gpu.max():.3e}; cpu fp32 median {q(e_cpu, .5):.3e} p90 "
           f"{q(e_cpu, .9):.3e} max {e_cpu.max():.3e}", flush=True)
     assert q(e_gpu, 0.5) <= 2 * q(e_cpu, 0.5) + 1e-6
@@ -163,13 +163,13 @@ def test_cli_double_precision_cpu_equals_gpu(gpu, tmp_path):
     assert lines[0].startswith(" CPUL [0][0] x: %g" % float(w.seeds[0, 0]))
 
 
-def test_full_config1_fp32_vs_fp64_kernel(gpu, orc):
+def test_full_lines_64k_fp32_vs_fp64_kernel(gpu, orc):
     """The headline workload's charges and all 65,536 lines, first 250 rows,
     angle rule (all steps), on both kernels.  The FP32 kernel's distance to the
     FP64 kernel's polylines is no larger than the CPU FP32 path's (256 sampled
     lines; prefixes before the first approach within 0.05 of a charge)."""
     from helpers import prefix_lengths
-    w = gen.config1()
+    w = gen.lines_64k()
     length = 250
     rule = rules.angle(stop=False)
     q = torch.from_numpy(w.charges).cuda()
@@ -190,7 +190,7 @@ def test_full_config1_fp32_vs_fp64_kernel(gpu, orc):
     first_all, _ = prefix_lengths(r64f, w.charges)
     allerr = orc.path_dist(r32, r64f, first_all).max(axis=0)
     q_ = lambda e, f: float(np.quantile(e, f))  # noqa: E731
-    print(f"[config1 250 rows vs FP64 kernel] sampled {idx.size}: gpu path median "
+    print(f"[lines_64k 250 rows vs FP64 kernel] sampled {idx.size}: gpu path median "
           f"{q_(e_gpu, .5):.2e} p90 {q_(e_gpu, .9):.2e}; cpu path median {q_(e_cpu, .5):.2e} "
           f"p90 {q_(e_cpu, .9):.2e}; all {w.n} lines: path median {q_(allerr, .5):.2e} "
           f"p99 {q_(allerr, .99):.2e}", flush=True)
@@ -198,9 +198,9 @@ def test_full_config1_fp32_vs_fp64_kernel(gpu, orc):
     assert q_(e_gpu, 0.9) <= 2 * q_(e_cpu, 0.9) + 1e-6
 
 
-def test_full_config1_field_vs_fp64(gpu, orc):
-    """Per-evaluation field with all 16,384 config-1 charges vs the FP64 sum."""
-    w = gen.config1()
+def test_full_lines_64k_field_vs_fp64(gpu, orc):
+    """Per-evaluation field with all 16,384 lines_64k charges vs the FP64 sum."""
+    w = gen.lines_64k()
     rng = np.random.default_rng(7)
     pts = np.concatenate([rng.uniform(-55, 55, (3000, 3)),
                           w.seeds[rng.integers(0, w.n, 1000)].astype(np.float64)]).astype(np.float32)
@@ -215,7 +215,7 @@ def test_full_config1_field_vs_fp64(gpu, orc):
     n64 = np.linalg.norm(E64, axis=1)
     rel = np.linalg.norm(E - E64, axis=1) / n64
     ok = n64 >= 0.1 * mag
-    print(f"[config1 field] {ok.sum()} well-conditioned of {len(pts)}: max rel {rel[ok].max():.3g}, "
+    print(f"[lines_64k field] {ok.sum()} well-conditioned of {len(pts)}: max rel {rel[ok].max():.3g}, "
           f"median {np.median(rel[ok]):.3g}; all max |dE|/sum|E_i| "
           f"{(np.linalg.norm(E - E64, axis=1) / mag).max():.3g}", flush=True)
     assert ok.sum() > 500

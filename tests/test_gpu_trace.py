@@ -89,9 +89,9 @@ def test_gpu_matches_reference_golden(gpu, name):
 @pytest.mark.parametrize("rule", [rules.ref(10.0), rules.plain(10.0), rules.angle(),
                                   rules.angle(stop=False)],
                          ids=["ref", "plain", "angle", "angle-nostop"])
-def test_config0_lines_vs_cpu_path(gpu, orc, rule):
-    """Config 0 charges (1,000 in [-10,10]^3), 512 of its 8,192 lines, 2,500 rows."""
-    w = gen.config0()
+def test_lines_8k_vs_cpu_path(gpu, orc, rule):
+    """lines_8k charges (1,000 in [-10,10]^3), 512 of its 8,192 lines, 2,500 rows."""
+    w = gen.lines_8k()
     idx = np.arange(0, w.n, 16)
     seeds = w.seeds[idx]
     rows_t, steps_t = gpu.trace(dev(w.charges), dev(seeds), w.len, rule, with_steps=True)
@@ -103,17 +103,17 @@ def test_config0_lines_vs_cpu_path(gpu, orc, rule):
     r64 = rows64_to_soa(orc.trace64(w.charges, seeds, w.len, r64_rule))
     assert np.isfinite(g_rows).all()
     same = g_steps == c_steps.astype(np.int64)
-    print(f"[steps] config0/{rules.NAMES[rule.rule]}: identical step counts {same.sum()}/{len(same)}; "
+    print(f"[steps] lines_8k/{rules.NAMES[rule.rule]}: identical step counts {same.sum()}/{len(same)}; "
           f"mean steps {g_steps.mean():.1f} (stop={rule.stop})", flush=True)
     if not rule.stop:
         assert (g_steps == w.len - 1).all()
     assert same.mean() >= 0.99
-    check_against_cpu(f"config0/{rules.NAMES[rule.rule]}", g_rows, c_rows, r64, w.charges, rule)
+    check_against_cpu(f"lines_8k/{rules.NAMES[rule.rule]}", g_rows, c_rows, r64, w.charges, rule)
 
 
 def test_field_matches_fp64(gpu, orc):
     """Per-evaluation field vs FP64: <= 1e-5 relative where not cancellation-dominated."""
-    w = gen.config0()
+    w = gen.lines_8k()
     rng = np.random.default_rng(3)
     pts = np.concatenate([rng.uniform(-12, 12, (4000, 3)),
                           w.seeds[:2000].astype(np.float64)]).astype(np.float32)

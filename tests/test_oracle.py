@@ -246,10 +246,10 @@ def test_restated_angle_f32_tracks_fp64(orc):
 
 def test_generator_is_pinned():
     """Seeded workloads are fixed by value (SplitMix64 streams, emtrace/gen.py)."""
-    w0 = gen.config0()
+    w0 = gen.lines_8k()
     assert w0.charges.shape == (1000, 4) and w0.seeds.shape == (8192, 3) and w0.len == 2500
-    assert gen.fingerprint(w0.charges, w0.seeds) == gen.fingerprint(gen.config0().charges,
-                                                                     gen.config0().seeds)
+    assert gen.fingerprint(w0.charges, w0.seeds) == gen.fingerprint(gen.lines_8k().charges,
+                                                                     gen.lines_8k().seeds)
     assert np.all(np.abs(w0.charges[:, :3]) <= 10.0)
     mag = np.abs(w0.charges[:, 3])
     assert mag.min() >= 0.5 and mag.max() <= 1.0
@@ -257,7 +257,7 @@ def test_generator_is_pinned():
     # 0.1 from the chosen source (a second charge may sit even closer)
     assert (seeds_d <= 0.1 + 2e-6).all() and (np.abs(seeds_d - 0.1) < 2e-6).mean() > 0.999
     assert gen.fingerprint(w0.charges, w0.seeds) == PINNED_CFG0
-    w1 = gen.config1()
+    w1 = gen.lines_64k()
     assert w1.interactions == 16384 * 65536 * 2499
     assert gen.fingerprint(w1.charges, w1.seeds) == PINNED_CFG1
 
