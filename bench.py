@@ -254,9 +254,10 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
             "mean_steps": taken / w.n,
             "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
                        "block": launch_s.block},
-            "note": "lines end at sinks; a wave (a block, when the charges are re-staged) stops "
-                    "evaluating once all its lines have ended, so the tuned shape has few lines per "
-                    "wave and one-wave blocks"}
+            "lines_past_first_phase": int((st > 192).sum().item()),
+            "note": "lines end at sinks; two phases: lane groups take a new line whenever theirs "
+                    "ends, up to 192 steps (launch = this phase's shape), then one 256-lane "
+                    "workgroup per line still moving; rows after a line's end are then filled"}
         del rows
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
     if on("batch_512"):

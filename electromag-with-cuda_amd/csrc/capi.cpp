@@ -91,9 +91,22 @@ static uint32_t tile_for(const DeviceInfo& dev, uint64_t p, int S, int resident)
 }
 
 int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
-                 bool stop)
+                 bool stop, bool refill)
 {
     Shape s;
+    if (refill) {
+        // first phase of the two-phase line-ends trace: a group takes a new
+        // line as soon as its line ends, so blocks stay full and may be
+        // large; wide groups keep the slowest line of the phase short
+        int S = 1;
+        while (S < 32 && p / (2ull * S) >= 64) S *= 2;
+        s.S = (req && req->lanes_per_line) ? req->lanes_per_line : S;
+        s.block = (req && req->block) ? req->block : 512;
+        s.tile = (req && req->tile) ? (uint32_t)req->tile : tile_for(dev, p, s.S, 2);
+        EM_TRY(validate_shape(dev, s));
+        out = s;
+        return EM_OK;
+    }
     // With line ends a wave keeps evaluating until its last line has ended (a
     // whole block, when the charges are re-staged tile by tile): few lines per
     // wave, and one-wave blocks over small tiles when the charges do not fit.
@@ -225,7 +238,8 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    EM_TRY(choose_shape(dev, p, n, launch, sh, st->stop != 0));
+    const bool ends = st->stop != 0;  // no checkpoints here: the two-phase line-ends trace
+    EM_TRY(choose_shape(dev, p, n, launch, sh, ends, ends));
 
     std::lock_guard<std::mutex> lk(g_mu);
     HostPathBuffers& b = g_hp[dev.device];
@@ -257,8 +271,12 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     cfg.len = (uint32_t)len;
     const StepP sp = to_stepp(*st);
     uint32_t* dsteps = steps ? static_cast<uint32_t*>(b.steps) : nullptr;
-    EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
-                               nullptr, 1, dsteps, b.stream));
+    if (ends)
+        EM_TRY(launch_trace_ends(sh, dev, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
+                                 dsteps, b.stream));
+    else
+        EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
+                                   nullptr, 1, dsteps, b.stream));
     EM_HIP(hipEventRecord(b.ev[2], b.stream));
     EM_HIP(hipEventSynchronize(b.ev[2]));
     const double td = now_s();
@@ -340,6 +358,7 @@ int em_shutdown(void)
     }
     EM_HIP(hipSetDevice(cur));
     EM_TRY(staging_release_all());
+    EM_TRY(ends_release_all());
     return adj_release_all();
 }
 
@@ -380,12 +399,18 @@ int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, f
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    EM_TRY(choose_shape(dev, p, n, launch, sh, step->stop != 0));
+    // line ends without checkpoints: the two-phase trace; with checkpoints
+    // the one-kernel path, whose steps the adjoint's recompute repeats
+    const bool ends = step->stop && !d_ckpt;
+    EM_TRY(choose_shape(dev, p, n, launch, sh, step->stop != 0, ends));
     Cfg cfg{};
     cfg.pitch = pitch;
     cfg.p = (uint32_t)p;
     cfg.n = (uint32_t)n;
     cfg.len = (uint32_t)len;
+    if (ends)
+        return launch_trace_ends(sh, dev, cfg, reinterpret_cast<const float4*>(d_charges), d_x, d_y,
+                                 d_z, to_stepp(*step), d_steps, static_cast<hipStream_t>(stream));
     return launch_trace_direct(sh, cfg, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
                                to_stepp(*step), d_ckpt, ckpt_every ? ckpt_every : 1, d_steps,
                                static_cast<hipStream_t>(stream));

@@ -60,14 +60,26 @@ struct Shape {
 // unsupported request.
 // `stop`: the angle rule with line ends (favours wide line groups and small
 // blocks); the forward and the adjoint of one problem must choose alike.
+// `refill`: the first phase of the two-phase line-ends trace (launch_trace_ends).
 int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
-                 bool stop = false);
+                 bool stop = false, bool refill = false);
 int validate_shape(const DeviceInfo& dev, const Shape& s);
 
 StepP to_stepp(const em_step& st);
 int validate_step(const em_step* st);
 
-// ---- kernels (defined in trace.hip / adjoint.hip) --------------------------
+// Dynamic LDS above 64 KiB (up to the 160 KiB a gfx950 workgroup may hold)
+// is requested explicitly per kernel.
+template <typename K>
+int allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 65536)
+        EM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return EM_OK;
+}
+
+// ---- kernels (defined in trace.hip / ends.hip / adjoint.hip) ---------------
 int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                         float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
                         uint32_t* steps, hipStream_t stream);
@@ -76,6 +88,14 @@ int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const It
                             float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
                             uint32_t* steps, hipStream_t stream);
 int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid);
+// The angle rule with line ends and no checkpoints, in two phases (ends.hip):
+// lane groups that take a new line whenever theirs ends, up to a step cap;
+// then one 256-lane workgroup per line still moving; then a coalesced fill of
+// the rows after each line's end.  `steps` may be NULL.
+int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
+                      float* x, float* y, float* z, const StepP& sp, uint32_t* steps,
+                      hipStream_t stream);
+int ends_release_all();
 int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,
                  const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,
                  float* ez, hipStream_t stream);

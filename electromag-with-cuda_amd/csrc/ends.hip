@@ -1,0 +1,426 @@
+// ends.hip -- the angle rule with line ends, traced in two phases.
+//
+// With line ends most lines of a problem are short (a line that runs into a
+// sink ends there) and a few never end (they escape and take every row).  In
+// the 16,384-charge, 65,536-line problem the median line ends after 52 steps
+// while 1.8 % of the lines run all 2,499 and hold 46 % of the steps.  One
+// launch shape cannot serve both: groups of S lanes per line keep a wave busy
+// until its slowest line ends, and a line that never ends is then traced at
+// the rate of S lanes.  Here:
+//   1. k_trace_ends<S>: a persistent grid of S-lane groups.  A group whose
+//      line ends takes the next line from a counter; a line still moving
+//      after kEndsCap steps is set aside (its state goes to a list) and the
+//      group moves on.  Charges go through LDS tiles as in the other kernels.
+//   2. k_trace_tail: one 256-lane workgroup (4 waves, one per SIMD) per line
+//      set aside; the 256 lanes split the charge sum of every step and read
+//      the charges straight from a padded copy in global memory (it stays in
+//      L2); wave 0 takes the step.
+//   3. k_ends_fill: rows after a line's end get its end point (the reference
+//      convention of the one-kernel path: an ended line keeps its last point),
+//      written row-major and coalesced.
+// Every line takes the same steps with the same arithmetic as in the
+// one-kernel path; only the split of each charge sum over lanes differs, so
+// results agree with it to FP32 summation rounding.
+#include <mutex>
+
+#include "em_internal.hpp"
+
+namespace em {
+
+namespace {
+
+constexpr uint32_t kEndsCap = 192;     // steps a line may take in the first phase
+constexpr int kTailLanes = 256;        // lanes per line in the second phase
+constexpr int kTailWaves = kTailLanes / 64;
+constexpr uint32_t kTailPad = kTailLanes * kGroup;  // granule of the padded charge copy
+constexpr uint32_t kFillRows = 64;     // rows per fill thread
+
+struct EndsCtrl {
+    uint32_t next_line;  // first phase: next line to hand out
+    uint32_t nsurv;      // lines set aside at the cap
+    uint32_t unused0;
+    uint32_t unused;
+};
+
+// Zero the counters (a kernel, not a memset: the sequence may be captured
+// into a graph) and write the padded charge copy of the second phase.
+__global__ void k_ends_prepare(EndsCtrl* __restrict__ ctrl, const float4* __restrict__ charges,
+                               uint32_t p, float4* __restrict__ padded, uint32_t pp)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctrl->next_line = 0u;
+        ctrl->nsurv = 0u;
+        ctrl->unused0 = 0u;
+        ctrl->unused = 0u;
+    }
+    if (i < pp) padded[i] = i < p ? charges[i] : make_float4(kFar, kFar, kFar, 0.f);
+}
+
+// First phase.  `surv` receives two float4 per set-aside line:
+// {x, y, z, h} and {ux, uy, uz, line index as bits}.
+template <int S>
+__global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __restrict__ charges,
+                                                    float* __restrict__ X, float* __restrict__ Y,
+                                                    float* __restrict__ Z, StepP sp,
+                                                    uint32_t tile_cap, uint32_t cap,
+                                                    uint32_t* __restrict__ steps,
+                                                    EndsCtrl* __restrict__ ctrl,
+                                                    float4* __restrict__ surv)
+{
+    extern __shared__ float4 tile[];
+    const int k = threadIdx.x % S;
+    const int leader = (int)(threadIdx.x % 64) - k;  // wave lane of the group's lane 0
+    const uint32_t p = cfg.p;
+    const uint32_t pp = round_up(p, S * kGroup);
+    const bool single = pp <= tile_cap;
+    if (single) {
+        stage(tile, charges, p, pp);
+        __syncthreads();
+    }
+    LineState st;
+    st.px = st.py = st.pz = 0.f;
+    st.ux = st.uy = st.uz = 0.f;
+    st.Px = st.Py = st.Pz = 0.f;
+    st.h = sp.h0;
+    st.alive = false;
+    st.steps = 0;
+    uint32_t line = 0, s = 0;
+    bool active = false, drained = false;
+    for (;;) {
+        if (!active && !drained) {
+            // the whole group is here (the branch is uniform over it)
+            uint32_t got = 0;
+            if (k == 0) got = atomicAdd(&ctrl->next_line, 1u);
+            got = __shfl(got, leader, 64);
+            if (got < cfg.n) {
+                line = got;
+                const uint64_t b = cfg.line_off + line;
+                st.px = X[b];
+                st.py = Y[b];
+                st.pz = Z[b];
+                st.Px = st.px; st.Py = st.py; st.Pz = st.pz;
+                st.ux = st.uy = st.uz = 0.f;
+                st.h = sp.h0;
+                st.alive = true;
+                st.steps = 0;
+                s = 1;
+                active = true;
+            } else {
+                drained = true;
+            }
+        }
+        // a wave (the block, when the charges are re-staged with barriers)
+        // leaves once none of its groups holds a line
+        const int any = single ? __any(active) : __syncthreads_or(active);
+        if (__builtin_amdgcn_readfirstlane(any) == 0) break;  // uniform: a scalar branch
+        float ex, ey, ez;
+        field_at<S>(tile, tile_cap, charges, p, k, st.px, st.py, st.pz, ex, ey, ez);
+        if (active) {
+            step_update(sp, s, ex * sp.scale, ey * sp.scale, ez * sp.scale, st);
+            if (st.alive) {
+                st.steps = s;
+                if (k == 0) {
+                    const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + line;
+                    X[o] = st.px;
+                    Y[o] = st.py;
+                    Z[o] = st.pz;
+                }
+            }
+            if (!st.alive || s + 1 >= cfg.len) {
+                if (k == 0) steps[cfg.line_index + line] = st.steps;
+                active = false;
+            } else if (s == cap) {
+                if (k == 0) {
+                    const uint32_t i = atomicAdd(&ctrl->nsurv, 1u);
+                    surv[2ull * i] = make_float4(st.px, st.py, st.pz, st.h);
+                    surv[2ull * i + 1] = make_float4(st.ux, st.uy, st.uz, __uint_as_float(line));
+                }
+                active = false;
+            } else {
+                ++s;
+            }
+        }
+    }
+}
+
+// One float4 record through a buffer load: a single lane-offset VGPR serves
+// every load of the walk (the record offsets go in scalar registers), which
+// leaves room for five workgroups of the second phase per CU; a load beyond
+// the buffer returns zeros instead of faulting.
+__device__ __forceinline__ float4 buffer_record(__amdgpu_buffer_rsrc_t rsrc, uint32_t lane_off,
+                                                uint32_t rec)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, lane_off, rec * 16u, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+}
+
+// Coulomb sum over pp padded records (a multiple of kTailPad) split over the
+// kTailLanes lanes of a workgroup: the software pipeline of pipelined_tile
+// (two register sets of kUnroll records, the loads of one set in flight while
+// the other is consumed), reading global memory.
+__device__ __forceinline__ void tail_sum(__amdgpu_buffer_rsrc_t rsrc, uint32_t pp, int tid,
+                                         FieldBody& body)
+{
+    constexpr uint32_t kStride = kTailLanes * kUnroll;  // records per group
+    constexpr uint32_t kFoldEvery = kBlk / kUnroll;
+    const uint32_t ngroups = pp / kStride;  // even, >= 2
+    const uint32_t lane_off = (uint32_t)tid * 16u;
+    float4 a[kUnroll], b[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) a[u] = buffer_record(rsrc, lane_off, u * kTailLanes);
+    for (uint32_t g = 0; g < ngroups; g += 2) {
+        const uint32_t rb = (g + 1) * kStride;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) b[u] = buffer_record(rsrc, lane_off, rb + u * kTailLanes);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) body.term(&a[u]);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t ra = min(g + 2, ngroups - 2) * kStride;  // the last pair re-reads itself
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) a[u] = buffer_record(rsrc, lane_off, ra + u * kTailLanes);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) body.term(&b[u]);
+        __builtin_amdgcn_sched_barrier(0);
+        if ((g + 2) % kFoldEvery == 0 || g + 2 >= ngroups) body.fold();
+    }
+}
+
+// Second phase: lines set aside at step `cap`, one workgroup per line; block b
+// takes set-aside lines b, b + grid, b + 2 grid, ...  The assignment is static
+// on purpose: a loop whose head is a branch on the thread index (thread 0
+// fetching the next item from a counter) and whose body holds barriers can be
+// threaded by the compiler into a loop that the other threads re-enter
+// without thread 0, which then never publishes the next item.
+// `padded` holds pp records (a multiple of kTailPad), charges then zero-charge
+// padding; lane t sums records t, t + 256, t + 512, ...
+__global__ void __launch_bounds__(kTailLanes) __attribute__((amdgpu_waves_per_eu(5))) k_trace_tail(Cfg cfg, const float4* __restrict__ padded,
+                                                           uint32_t pp, float* __restrict__ X,
+                                                           float* __restrict__ Y, float* __restrict__ Z,
+                                                           StepP sp, uint32_t cap,
+                                                           uint32_t* __restrict__ steps,
+                                                           EndsCtrl* __restrict__ ctrl,
+                                                           const float4* __restrict__ surv)
+{
+    __shared__ float4 part[kTailWaves];
+    __shared__ float4 next;  // the point after the step; w = 1 while the line moves
+    const int tid = threadIdx.x;
+    const int wave = tid / 64;
+    const uint32_t nsurv = ctrl->nsurv;  // final: the first phase has completed
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float4*>(padded), (short)0, (int)(pp * sizeof(float4)), 0x00020000);
+    for (uint32_t it = blockIdx.x; it < nsurv; it += gridDim.x) {
+        const float4 a = surv[2ull * it];
+        const float4 b = surv[2ull * it + 1];
+        const uint32_t line = __float_as_uint(b.w);
+        LineState st;
+        st.px = a.x; st.py = a.y; st.pz = a.z; st.h = a.w;
+        st.ux = b.x; st.uy = b.y; st.uz = b.z;
+        st.Px = st.Py = st.Pz = 0.f;
+        st.alive = true;
+        st.steps = cap;
+        for (uint32_t s = cap + 1; s < cfg.len; ++s) {
+            FieldBody body;
+            body.init(st.px, st.py, st.pz);
+            tail_sum(rsrc, pp, tid, body);
+            const float ex = group_sum<64>(body.acc.sx + body.acc.cx);
+            const float ey = group_sum<64>(body.acc.sy + body.acc.cy);
+            const float ez = group_sum<64>(body.acc.sz + body.acc.cz);
+            if ((tid & 63) == 0) part[wave] = make_float4(ex, ey, ez, 0.f);
+            __syncthreads();
+            if (wave == 0) {
+                float4 e = part[0];
+#pragma unroll
+                for (int w = 1; w < kTailWaves; ++w) {
+                    const float4 q = part[w];
+                    e.x += q.x;
+                    e.y += q.y;
+                    e.z += q.z;
+                }
+                step_update(sp, s, e.x * sp.scale, e.y * sp.scale, e.z * sp.scale, st);
+                if (tid == 0) next = make_float4(st.px, st.py, st.pz, st.alive ? 1.f : 0.f);
+            }
+            __syncthreads();
+            const float4 q = next;
+            // the line ended at x_{s-1}; the flag is the same in every thread
+            // (read from LDS), taken into a scalar register so the exit is
+            // a uniform branch
+            if (__builtin_amdgcn_readfirstlane(__float_as_uint(q.w)) == 0u) break;
+            st.px = q.x;
+            st.py = q.y;
+            st.pz = q.z;
+            st.steps = s;
+            if (tid == 0) {
+                const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + line;
+                X[o] = st.px;
+                Y[o] = st.py;
+                Z[o] = st.pz;
+            }
+        }
+        if (tid == 0) steps[cfg.line_index + line] = st.steps;
+    }
+}
+
+// Rows steps+1 .. len-1 of every line get row `steps` (the end point).
+__global__ void k_ends_fill(Cfg cfg, float* __restrict__ X, float* __restrict__ Y,
+                            float* __restrict__ Z, const uint32_t* __restrict__ steps)
+{
+    const uint32_t line = blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= cfg.n) return;
+    const uint32_t e = steps[cfg.line_index + line];
+    const uint32_t r0 = max(e + 1u, blockIdx.y * kFillRows);
+    const uint32_t r1 = min(cfg.len, (blockIdx.y + 1u) * kFillRows);
+    if (r0 >= r1) return;
+    const uint64_t b = cfg.line_off + line;
+    const uint64_t src = b + (uint64_t)e * cfg.pitch;
+    const float x = X[src], y = Y[src], z = Z[src];
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint64_t o = b + (uint64_t)r * cfg.pitch;
+        X[o] = x;
+        Y[o] = y;
+        Z[o] = z;
+    }
+}
+
+template <int S>
+int launch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
+                  float* x, float* y, float* z, const StepP& sp, uint32_t* steps, EndsCtrl* ctrl,
+                  float4* surv, hipStream_t stream)
+{
+    const size_t lds = (size_t)sh.tile * sizeof(float4);
+    EM_TRY(allow_lds(k_trace_ends<S>, lds));
+    int per_cu = 0;
+    EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_ends<S>, sh.block, lds));
+    if (per_cu < 1)
+        EM_FAIL(EM_ERR_INVALID, "launch shape does not fit a CU (block %d, tile %u)", sh.block, sh.tile);
+    const uint64_t groups_per_block = (uint64_t)sh.block / S;
+    const uint64_t want = (cfg.n + groups_per_block - 1) / groups_per_block;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * dev.cus, want);
+    hipLaunchKernelGGL(k_trace_ends<S>, dim3(grid), dim3(sh.block), lds, stream, cfg, charges, x, y,
+                       z, sp, sh.tile, kEndsCap, steps, ctrl, surv);
+    EM_HIP(hipGetLastError());
+    return EM_OK;
+}
+
+int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
+                    float* x, float* y, float* z, const StepP& sp, uint32_t* steps, EndsCtrl* ctrl,
+                    float4* surv, hipStream_t stream)
+{
+    switch (sh.S) {
+        case 1: return launch_phase1<1>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        case 2: return launch_phase1<2>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        case 4: return launch_phase1<4>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        case 8: return launch_phase1<8>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        case 16: return launch_phase1<16>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        case 32: return launch_phase1<32>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        default: EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1,2,4,8,16,32}", sh.S);
+    }
+}
+
+// Per-device work memory of the two-phase trace: counters, step counts (when
+// the caller keeps none), the set-aside list and the padded charge copy.
+// Calls on one device use it in turn: each call's work waits for the previous
+// call's (an event), except under stream capture, where ordering is the
+// capturing caller's.
+struct EndsScratch {
+    std::mutex mu;
+    int device = -1;
+    char* buf = nullptr;
+    size_t bytes = 0;
+    hipEvent_t done = nullptr;
+    bool recorded = false;
+};
+EndsScratch g_ends[64];
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
+                      float* x, float* y, float* z, const StepP& sp, uint32_t* steps,
+                      hipStream_t stream)
+{
+    if (dev.device < 0 || dev.device >= 64) EM_FAIL(EM_ERR_INVALID, "device %d", dev.device);
+    const uint32_t pp = std::max<uint32_t>(kTailPad, round_up(cfg.p, kTailPad));
+    const size_t off_steps = align256(sizeof(EndsCtrl));
+    const size_t steps_bytes = steps ? 0 : align256((size_t)cfg.n * sizeof(uint32_t));
+    const size_t off_surv = off_steps + steps_bytes;
+    const size_t off_pad = off_surv + align256((size_t)cfg.n * 2 * sizeof(float4));
+    const size_t need = off_pad + (size_t)pp * sizeof(float4);
+
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    EM_HIP(hipStreamIsCapturing(stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+
+    EndsScratch& w = g_ends[dev.device];
+    std::lock_guard<std::mutex> lk(w.mu);
+    w.device = dev.device;
+    if (!w.done) EM_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+    if (w.bytes < need) {
+        if (capturing)
+            EM_FAIL(EM_ERR_INVALID,
+                    "line-ends work memory must grow to %zu bytes: make a first call of this "
+                    "size outside stream capture", need);
+        if (w.recorded) EM_HIP(hipEventSynchronize(w.done));
+        if (w.buf) EM_HIP(hipFree(w.buf));
+        w.buf = nullptr;
+        w.bytes = 0;
+        EM_HIP(hipMalloc(reinterpret_cast<void**>(&w.buf), need));
+        w.bytes = need;
+    }
+    if (w.recorded && !capturing) EM_HIP(hipStreamWaitEvent(stream, w.done, 0));
+
+    EndsCtrl* ctrl = reinterpret_cast<EndsCtrl*>(w.buf);
+    uint32_t* st = steps ? steps : reinterpret_cast<uint32_t*>(w.buf + off_steps);
+    float4* surv = reinterpret_cast<float4*>(w.buf + off_surv);
+    float4* padded = reinterpret_cast<float4*>(w.buf + off_pad);
+
+    hipLaunchKernelGGL(k_ends_prepare, dim3((pp + 255) / 256), dim3(256), 0, stream, ctrl, charges,
+                       cfg.p, padded, pp);
+    EM_HIP(hipGetLastError());
+    EM_TRY(dispatch_phase1(sh, dev, cfg, charges, x, y, z, sp, st, ctrl, surv, stream));
+    if (cfg.len > kEndsCap + 1) {
+        int per_cu = 0;
+        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_tail, kTailLanes, 0));
+        if (per_cu < 1) EM_FAIL(EM_ERR_INVALID, "second-phase workgroup does not fit a CU");
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * dev.cus, cfg.n);
+        hipLaunchKernelGGL(k_trace_tail, dim3(grid), dim3(kTailLanes), 0, stream, cfg, padded, pp, x,
+                           y, z, sp, kEndsCap, st, ctrl, surv);
+        EM_HIP(hipGetLastError());
+    }
+    const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
+    if (fill_grid.y > 65535u) EM_FAIL(EM_ERR_INVALID, "len %u exceeds the fill grid", cfg.len);
+    hipLaunchKernelGGL(k_ends_fill, fill_grid, dim3(256), 0, stream, cfg, x, y, z, st);
+    EM_HIP(hipGetLastError());
+    if (!capturing) {
+        EM_HIP(hipEventRecord(w.done, stream));
+        w.recorded = true;
+    }
+    return EM_OK;
+}
+
+int ends_release_all()
+{
+    int rc = EM_OK;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return EM_ERR_HIP;
+    for (auto& w : g_ends) {
+        std::lock_guard<std::mutex> lk(w.mu);
+        if (!w.buf && !w.done) continue;
+        if (hipSetDevice(w.device) != hipSuccess) rc = EM_ERR_HIP;
+        if (w.recorded && hipEventSynchronize(w.done) != hipSuccess) rc = EM_ERR_HIP;
+        if (w.buf && hipFree(w.buf) != hipSuccess) rc = EM_ERR_HIP;
+        if (w.done && hipEventDestroy(w.done) != hipSuccess) rc = EM_ERR_HIP;
+        w.buf = nullptr;
+        w.bytes = 0;
+        w.done = nullptr;
+        w.recorded = false;
+    }
+    if (hipSetDevice(cur) != hipSuccess) rc = EM_ERR_HIP;
+    return rc;
+}
+
+}  // namespace em

@@ -158,17 +158,6 @@ __global__ void __launch_bounds__(512) k_field(const float4* __restrict__ charge
     }
 }
 
-// Dynamic LDS above 64 KiB (up to the 160 KiB a gfx950 workgroup may hold)
-// is requested explicitly per kernel.
-template <typename K>
-int allow_lds(K kernel, size_t bytes)
-{
-    if (bytes > 65536)
-        EM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    return EM_OK;
-}
-
 template <template <int> class F, typename... A>
 int dispatch_S(int S, A&&... a)
 {

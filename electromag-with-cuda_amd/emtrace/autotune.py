@@ -66,10 +66,11 @@ def _device_key() -> str:
 # local cache has no entry: the fastest shape of each sweep, ties (within
 # 1%) going to fewer lanes per line.  [S, block, tile, ms at 48 rows]
 SHIPPED = {
-    # 16,384 charges, 65,536 lines, angle rule (all steps / with line ends:
-    # 210 ms vs 917 ms for the all-steps shape)
+    # 16,384 charges, 65,536 lines, angle rule: all steps; with line ends the
+    # first phase of the two-phase trace (56.5 ms; 70 ms with one-wave blocks
+    # over 512-charge tiles, 76 ms with 8 lanes per line)
     "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, None],
-    "plain:gfx950:256:16384:65536:2:1": [32, 64, 512, None],
+    "plain:gfx950:256:16384:65536:2:1": [32, 512, 4096, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule
     "batch:gfx950:256:512:8192:1048576:2:0": [4, 256, 4096, None],
 }
@@ -102,10 +103,10 @@ def candidates(p: int, n: int, cus: int, batch: bool = False,
     Single configuration: S grows until the grid has 8 waves per SIMD.  Batch:
     S in {1, 2, 4, 8} always -- a smaller line block per work item evens out
     the persistent grid's finishing times even when waves are plentiful.
-    With line ends (stop): S up to 32 and blocks down to one wave whatever the
-    line count -- a wave (with re-staged charges, a block) runs until its last
-    line ends, so fewer lines per wave and per block waste fewer lanes -- and
-    smaller tiles, which keep several one-wave blocks resident per CU."""
+    With line ends (stop) the shape is that of the first phase of the
+    two-phase trace, where a lane group takes a new line as soon as its line
+    ends: S from 4 up to 32 (wide groups shorten the phase's last lines),
+    blocks from one wave up, and smaller tiles as well."""
     out = []
     for S in (1, 2, 4, 8, 16, 32):
         if batch:

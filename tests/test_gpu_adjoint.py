@@ -38,6 +38,19 @@ def forward_and_adjoint(em, charges, seeds, length, rule, G, every):
     return rows.cpu().numpy(), gc.cpu().numpy().astype(np.float64), gs.cpu().numpy().T.astype(np.float64)
 
 
+def forward_steps(em, charges, seeds, length, rule, every=50):
+    """Step counts of the checkpointed forward (the one-kernel path that the
+    adjoint's recompute repeats; without checkpoints, line ends are traced in
+    two phases whose charge sums are split differently past step 192)."""
+    n = seeds.shape[0]
+    rows = em.seed_rows(dev(seeds), length)
+    ck = torch.empty(max(1, em.ckpt_size(n, length, every)), dtype=torch.float32, device="cuda")
+    steps = torch.empty(n, dtype=torch.int32, device="cuda")
+    em.trace_into(dev(charges), rows, n, rule, None, ck, every, steps=steps)
+    torch.cuda.synchronize()
+    return steps.cpu().numpy().astype(np.int64)
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
@@ -50,8 +63,7 @@ def test_adjoint_matches_fp64_adjoint_small(gpu, orc, rule):
     w = gen.plain_config(40, 96, 4.0, 60, seed=21)
     rng = np.random.default_rng(0)
     G = rng.normal(size=(3, w.len, w.n))
-    _, g_steps = gpu.trace(dev(w.charges), dev(w.seeds), w.len, rule, with_steps=True)
-    g_steps = g_steps.cpu().numpy().astype(np.int64)
+    g_steps = forward_steps(gpu, w.charges, w.seeds, w.len, rule, 7)
     rows, gc, gs = forward_and_adjoint(gpu, w.charges, w.seeds, w.len, rule, G, 7)
     G64 = np.ascontiguousarray(G.transpose(1, 2, 0))
     gp, gsp = orc.adjoint64_path(w.charges, w.seeds, w.len, rule, G64, rows, g_steps)
@@ -123,8 +135,7 @@ def test_adjoint_2k_gradient_vs_fp64_adjoint_and_fd(gpu, orc):
     loss.backward()
     g_full = c.grad.cpu().numpy().astype(np.float64)
     # per-line agreement with FP64
-    _, g_steps = gpu.trace(dev(w.charges), dev(w.seeds), w.len, rule, with_steps=True)
-    g_steps = g_steps.cpu().numpy().astype(np.int64)
+    g_steps = forward_steps(gpu, w.charges, w.seeds, w.len, rule, 50)
     r64, s64 = orc.trace64(w.charges, w.seeds, w.len, rule, with_steps=True)
     g_rows = rows.detach().cpu().numpy()
     ok = agreeing_lines(g_rows, g_steps, rows64_to_soa(r64), s64.astype(np.int64))
