@@ -131,7 +131,13 @@ int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x,
  * CL_Electrostatics.cpp:396-513 MainFunctor; Electrostatics.cl.c:282-367).
  * d_ckpt (may be NULL): when non-NULL, the forward also stores the state
  * needed by em_adjoint_dev_f32 every ckpt_every steps; size it with
- * em_ckpt_floats().  d_steps (may be NULL): n step counts. */
+ * em_ckpt_floats().  d_steps (may be NULL): n step counts.
+ * The angle rule with stop and no checkpoints is traced in two phases (lane
+ * groups that take a new line when theirs ends, then one workgroup per line
+ * still moving after 192 steps); `launch` then sets the first phase's shape.
+ * That path keeps per-device work memory, allocated on first use and grown as
+ * needed; under stream capture it must already be large enough (a call of
+ * the same or a larger size made outside capture), else EM_ERR_INVALID. */
 int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y,
                      float* d_z, size_t n, size_t len, size_t pitch,
                      const em_step* step, const em_launch* launch,
