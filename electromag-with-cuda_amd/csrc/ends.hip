@@ -11,10 +11,11 @@
 //      line ends takes the next line from a counter; a line still moving
 //      after kEndsCap steps is set aside (its state goes to a list) and the
 //      group moves on.  Charges go through LDS tiles as in the other kernels.
-//   2. k_trace_tail: one 256-lane workgroup (4 waves, one per SIMD) per line
-//      set aside; the 256 lanes split the charge sum of every step and read
-//      the charges straight from a padded copy in global memory (it stays in
-//      L2); wave 0 takes the step.
+//   2. k_trace_tail: two 256-lane workgroups (4 waves, one per SIMD) per CU;
+//      each owns a share of the lines set aside and traces up to 8 of them at
+//      once.  The 256 lanes split every charge sum; each charge record a lane
+//      loads (straight from a padded copy in global memory, which stays in
+//      L2) serves all the workgroup's lines; wave 0 takes the steps.
 //   3. k_ends_fill: rows after a line's end get its end point (the reference
 //      convention of the one-kernel path: an ended line keeps its last point),
 //      written row-major and coalesced.
@@ -30,9 +31,11 @@ namespace em {
 namespace {
 
 constexpr uint32_t kEndsCap = 192;     // steps a line may take in the first phase
-constexpr int kTailLanes = 256;        // lanes per line in the second phase
+constexpr int kTailLanes = 256;        // lanes of a second-phase workgroup
 constexpr int kTailWaves = kTailLanes / 64;
+constexpr int kSlots = 8;              // lines a second-phase workgroup traces at once
 constexpr uint32_t kTailPad = kTailLanes * kGroup;  // granule of the padded charge copy
+constexpr size_t kTailReserve = 64 * 1024;  // LDS that keeps two second-phase workgroups per CU
 constexpr uint32_t kFillRows = 64;     // rows per fill thread
 
 struct EndsCtrl {
@@ -145,9 +148,8 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
 }
 
 // One float4 record through a buffer load: a single lane-offset VGPR serves
-// every load of the walk (the record offsets go in scalar registers), which
-// leaves room for five workgroups of the second phase per CU; a load beyond
-// the buffer returns zeros instead of faulting.
+// every load of the walk (the record offsets go in scalar registers); a load
+// beyond the buffer returns zeros instead of faulting.
 __device__ __forceinline__ float4 buffer_record(__amdgpu_buffer_rsrc_t rsrc, uint32_t lane_off,
                                                 uint32_t rec)
 {
@@ -156,17 +158,30 @@ __device__ __forceinline__ float4 buffer_record(__amdgpu_buffer_rsrc_t rsrc, uin
                        __uint_as_float(v[3]));
 }
 
-// Coulomb sum over pp padded records (a multiple of kTailPad) split over the
-// kTailLanes lanes of a workgroup: the software pipeline of pipelined_tile
-// (two register sets of kUnroll records, the loads of one set in flight while
-// the other is consumed), reading global memory.
-__device__ __forceinline__ void tail_sum(__amdgpu_buffer_rsrc_t rsrc, uint32_t pp, int tid,
-                                         FieldBody& body)
+// Coulomb sums at the NACT points of a workgroup's active slots over pp padded
+// records (a multiple of kTailPad) split over the kTailLanes lanes: every
+// record a lane loads serves all NACT points.  The software pipeline of
+// pipelined_tile (two register sets of kUnroll records, one set's loads in
+// flight while the other is consumed), reading global memory.  Each lane
+// sums pp / kTailLanes records per point in plain FP32 (the blocked
+// compensation of the other kernels folds every 128 records, so it would
+// change nothing here); the lanes' sums are then combined by a butterfly and
+// across the waves in LDS.  Lane 0 of every wave writes its wave's sums to
+// part[wave][l].
+template <int NACT>
+__device__ __forceinline__ void tail_sums(__amdgpu_buffer_rsrc_t rsrc, uint32_t pp, int tid,
+                                          const float4* __restrict__ pts, float4* __restrict__ part)
 {
     constexpr uint32_t kStride = kTailLanes * kUnroll;  // records per group
-    constexpr uint32_t kFoldEvery = kBlk / kUnroll;
     const uint32_t ngroups = pp / kStride;  // even, >= 2
     const uint32_t lane_off = (uint32_t)tid * 16u;
+    float px[NACT], py[NACT], pz[NACT], ex[NACT], ey[NACT], ez[NACT];
+#pragma unroll
+    for (int l = 0; l < NACT; ++l) {
+        const float4 q = pts[l];
+        px[l] = q.x; py[l] = q.y; pz[l] = q.z;
+        ex[l] = ey[l] = ez[l] = 0.f;
+    }
     float4 a[kUnroll], b[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) a[u] = buffer_record(rsrc, lane_off, u * kTailLanes);
@@ -176,91 +191,195 @@ __device__ __forceinline__ void tail_sum(__amdgpu_buffer_rsrc_t rsrc, uint32_t p
         for (int u = 0; u < kUnroll; ++u) b[u] = buffer_record(rsrc, lane_off, rb + u * kTailLanes);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) body.term(&a[u]);
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int l = 0; l < NACT; ++l) coulomb(a[u], px[l], py[l], pz[l], ex[l], ey[l], ez[l]);
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t ra = min(g + 2, ngroups - 2) * kStride;  // the last pair re-reads itself
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) a[u] = buffer_record(rsrc, lane_off, ra + u * kTailLanes);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) body.term(&b[u]);
+        for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+            for (int l = 0; l < NACT; ++l) coulomb(b[u], px[l], py[l], pz[l], ex[l], ey[l], ez[l]);
         __builtin_amdgcn_sched_barrier(0);
-        if ((g + 2) % kFoldEvery == 0 || g + 2 >= ngroups) body.fold();
+    }
+    const int wave = tid / 64;
+#pragma unroll
+    for (int l = 0; l < NACT; ++l) {
+        const float sx = group_sum<64>(ex[l]);
+        const float sy = group_sum<64>(ey[l]);
+        const float sz = group_sum<64>(ez[l]);
+        if ((tid & 63) == 0) part[wave * kSlots + l] = make_float4(sx, sy, sz, 0.f);
     }
 }
 
-// Second phase: lines set aside at step `cap`, one workgroup per line; block b
-// takes set-aside lines b, b + grid, b + 2 grid, ...  The assignment is static
-// on purpose: a loop whose head is a branch on the thread index (thread 0
-// fetching the next item from a counter) and whose body holds barriers can be
-// threaded by the compiler into a loop that the other threads re-enter
-// without thread 0, which then never publishes the next item.
+// State of one slot of the second phase, held by lane `slot` of wave 0.
+struct Slot {
+    LineState st;
+    uint32_t line;
+    uint32_t s;  // the step this slot's line takes next
+};
+
+// Wave 0 only.  Slots [0, nact) hold lines; `done` marks those whose line
+// has just finished.  Finished and empty slots take the next lines of this
+// workgroup's list (set-aside lines blockIdx.x + k * gridDim.x, k = next_k,
+// next_k + 1, ...); the slots still holding lines are then packed to the
+// front through LDS (`xfer`), their points published in `pts` and their count
+// returned.
+__device__ __forceinline__ uint32_t refill_slots(Slot& sl, bool done, uint32_t nact,
+                                                 uint32_t& next_k, uint32_t nsurv, uint32_t cap,
+                                                 const float4* __restrict__ surv,
+                                                 float4* __restrict__ xfer, float4* __restrict__ pts)
+{
+    const uint32_t lane = threadIdx.x;  // wave 0: 0..63
+    const bool holds = lane < nact && !done;
+    const bool need = lane < (uint32_t)kSlots && !holds;
+    const uint64_t need_mask = __ballot(need);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need_mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)need_mask, 0u));
+    const uint32_t j = blockIdx.x + (next_k + rank) * gridDim.x;
+    const bool got = need && j < nsurv;
+    // the lines handed out are the list's next ones in rank order, so the
+    // list position advances by their count (and stops at the list's end)
+    next_k += (uint32_t)__popcll(__ballot(got));
+    if (got) {
+        const float4 a = surv[2ull * j];
+        const float4 b = surv[2ull * j + 1];
+        sl.st.px = a.x; sl.st.py = a.y; sl.st.pz = a.z; sl.st.h = a.w;
+        sl.st.ux = b.x; sl.st.uy = b.y; sl.st.uz = b.z;
+        sl.st.Px = sl.st.Py = sl.st.Pz = 0.f;
+        sl.st.alive = true;
+        sl.st.steps = cap;
+        sl.line = __float_as_uint(b.w);
+        sl.s = cap + 1;
+    }
+    const bool keep = holds || got;
+    const uint64_t keep_mask = __ballot(keep);
+    const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(keep_mask >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)keep_mask, 0u));
+    const uint32_t nkeep = (uint32_t)__popcll(keep_mask);
+    if (keep) {
+        xfer[3 * pos] = make_float4(sl.st.px, sl.st.py, sl.st.pz, sl.st.h);
+        xfer[3 * pos + 1] = make_float4(sl.st.ux, sl.st.uy, sl.st.uz, __uint_as_float(sl.line));
+        xfer[3 * pos + 2] = make_float4(__uint_as_float(sl.s), __uint_as_float(sl.st.steps), 0.f, 0.f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < nkeep) {
+        const float4 a = xfer[3 * lane];
+        const float4 b = xfer[3 * lane + 1];
+        const float4 c = xfer[3 * lane + 2];
+        sl.st.px = a.x; sl.st.py = a.y; sl.st.pz = a.z; sl.st.h = a.w;
+        sl.st.ux = b.x; sl.st.uy = b.y; sl.st.uz = b.z;
+        sl.st.alive = true;
+        sl.line = __float_as_uint(b.w);
+        sl.s = __float_as_uint(c.x);
+        sl.st.steps = __float_as_uint(c.y);
+        pts[lane] = make_float4(sl.st.px, sl.st.py, sl.st.pz, 0.f);
+    }
+    return nkeep;
+}
+
+// Second phase: the lines set aside at step `cap`.  Two workgroups per CU
+// (the launch reserves enough LDS that no third one fits): one wave per SIMD
+// each, so while one takes its steps (a serial stretch in wave 0 between two
+// barriers) the other sums.  Workgroup b owns set-aside lines b, b + grid,
+// b + 2 grid, ...; with workgroups b and b + grid/2 on one CU (the dispatch
+// order of a 1-D grid over the CUs), a CU holds every line congruent to b
+// modulo grid/2.  A workgroup traces up to kSlots of its lines at once, so
+// each charge record it loads from L2 serves every one of them.
+// The assignment is static on purpose: a loop whose head is a branch on the
+// thread index (one thread fetching the next item from a counter) and whose
+// body holds barriers can be threaded by the compiler into a loop that the
+// other threads re-enter without that thread, which then never publishes the
+// next item.  Loop exits on LDS values go through readfirstlane, so they are
+// scalar branches.
 // `padded` holds pp records (a multiple of kTailPad), charges then zero-charge
 // padding; lane t sums records t, t + 256, t + 512, ...
-__global__ void __launch_bounds__(kTailLanes) __attribute__((amdgpu_waves_per_eu(5))) k_trace_tail(Cfg cfg, const float4* __restrict__ padded,
+__global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4* __restrict__ padded,
                                                            uint32_t pp, float* __restrict__ X,
                                                            float* __restrict__ Y, float* __restrict__ Z,
                                                            StepP sp, uint32_t cap,
                                                            uint32_t* __restrict__ steps,
-                                                           EndsCtrl* __restrict__ ctrl,
+                                                           const EndsCtrl* __restrict__ ctrl,
                                                            const float4* __restrict__ surv)
 {
-    __shared__ float4 part[kTailWaves];
-    __shared__ float4 next;  // the point after the step; w = 1 while the line moves
+    __shared__ float4 part[kTailWaves * kSlots];
+    __shared__ float4 pts[kSlots];
+    __shared__ float4 xfer[3 * kSlots];
+    __shared__ uint32_t s_nact;
     const int tid = threadIdx.x;
     const int wave = tid / 64;
     const uint32_t nsurv = ctrl->nsurv;  // final: the first phase has completed
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float4*>(padded), (short)0, (int)(pp * sizeof(float4)), 0x00020000);
-    for (uint32_t it = blockIdx.x; it < nsurv; it += gridDim.x) {
-        const float4 a = surv[2ull * it];
-        const float4 b = surv[2ull * it + 1];
-        const uint32_t line = __float_as_uint(b.w);
-        LineState st;
-        st.px = a.x; st.py = a.y; st.pz = a.z; st.h = a.w;
-        st.ux = b.x; st.uy = b.y; st.uz = b.z;
-        st.Px = st.Py = st.Pz = 0.f;
-        st.alive = true;
-        st.steps = cap;
-        for (uint32_t s = cap + 1; s < cfg.len; ++s) {
-            FieldBody body;
-            body.init(st.px, st.py, st.pz);
-            tail_sum(rsrc, pp, tid, body);
-            const float ex = group_sum<64>(body.acc.sx + body.acc.cx);
-            const float ey = group_sum<64>(body.acc.sy + body.acc.cy);
-            const float ez = group_sum<64>(body.acc.sz + body.acc.cz);
-            if ((tid & 63) == 0) part[wave] = make_float4(ex, ey, ez, 0.f);
-            __syncthreads();
-            if (wave == 0) {
-                float4 e = part[0];
+    Slot sl;  // meaningful in lanes 0..kSlots-1 of wave 0
+    sl.st.px = sl.st.py = sl.st.pz = 0.f;
+    sl.st.ux = sl.st.uy = sl.st.uz = 0.f;
+    sl.st.Px = sl.st.Py = sl.st.Pz = 0.f;
+    sl.st.h = sp.h0;
+    sl.st.alive = false;
+    sl.st.steps = 0;
+    sl.line = 0;
+    sl.s = 0;
+    uint32_t next_k = 0, nact = 0;
+    if (wave == 0) {
+        nact = refill_slots(sl, false, 0u, next_k, nsurv, cap, surv, xfer, pts);
+        if (tid == 0) s_nact = nact;
+    }
+    for (;;) {
+        __syncthreads();  // pts and s_nact published
+        const uint32_t n_now = __builtin_amdgcn_readfirstlane(s_nact);
+        if (n_now == 0u) break;
+        switch (n_now) {
+            case 1: tail_sums<1>(rsrc, pp, tid, pts, part); break;
+            case 2: tail_sums<2>(rsrc, pp, tid, pts, part); break;
+            case 3: tail_sums<3>(rsrc, pp, tid, pts, part); break;
+            case 4: tail_sums<4>(rsrc, pp, tid, pts, part); break;
+            case 5: tail_sums<5>(rsrc, pp, tid, pts, part); break;
+            case 6: tail_sums<6>(rsrc, pp, tid, pts, part); break;
+            case 7: tail_sums<7>(rsrc, pp, tid, pts, part); break;
+            default: tail_sums<8>(rsrc, pp, tid, pts, part); break;
+        }
+        __syncthreads();  // part complete
+        if (wave == 0) {
+            bool done = false;
+            if ((uint32_t)tid < n_now) {
+                float4 e = part[tid];
 #pragma unroll
                 for (int w = 1; w < kTailWaves; ++w) {
-                    const float4 q = part[w];
+                    const float4 q = part[w * kSlots + tid];
                     e.x += q.x;
                     e.y += q.y;
                     e.z += q.z;
                 }
-                step_update(sp, s, e.x * sp.scale, e.y * sp.scale, e.z * sp.scale, st);
-                if (tid == 0) next = make_float4(st.px, st.py, st.pz, st.alive ? 1.f : 0.f);
+                step_update(sp, sl.s, e.x * sp.scale, e.y * sp.scale, e.z * sp.scale, sl.st);
+                if (sl.st.alive) {
+                    sl.st.steps = sl.s;
+                    const uint64_t o = cfg.line_off + (uint64_t)sl.s * cfg.pitch + sl.line;
+                    X[o] = sl.st.px;
+                    Y[o] = sl.st.py;
+                    Z[o] = sl.st.pz;
+                }
+                done = !sl.st.alive || sl.s + 1 >= cfg.len;
+                if (done) steps[cfg.line_index + sl.line] = sl.st.steps;
+                else ++sl.s;
             }
-            __syncthreads();
-            const float4 q = next;
-            // the line ended at x_{s-1}; the flag is the same in every thread
-            // (read from LDS), taken into a scalar register so the exit is
-            // a uniform branch
-            if (__builtin_amdgcn_readfirstlane(__float_as_uint(q.w)) == 0u) break;
-            st.px = q.x;
-            st.py = q.y;
-            st.pz = q.z;
-            st.steps = s;
-            if (tid == 0) {
-                const uint64_t o = cfg.line_off + (uint64_t)s * cfg.pitch + line;
-                X[o] = st.px;
-                Y[o] = st.py;
-                Z[o] = st.pz;
+            // slots change only when a line finished or an empty slot can
+            // take a line from the list; otherwise just publish the points
+            const bool change = __ballot(done) != 0ull ||
+                                (n_now < (uint32_t)kSlots && blockIdx.x + next_k * gridDim.x < nsurv);
+            if (change) {
+                nact = refill_slots(sl, done, n_now, next_k, nsurv, cap, surv, xfer, pts);
+            } else {
+                if ((uint32_t)tid < n_now) pts[tid] = make_float4(sl.st.px, sl.st.py, sl.st.pz, 0.f);
+                nact = n_now;
             }
+            if (tid == 0) s_nact = nact;
         }
-        if (tid == 0) steps[cfg.line_index + line] = st.steps;
     }
 }
 
@@ -383,12 +502,15 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
     EM_HIP(hipGetLastError());
     EM_TRY(dispatch_phase1(sh, dev, cfg, charges, x, y, z, sp, st, ctrl, surv, stream));
     if (cfg.len > kEndsCap + 1) {
+        // two workgroups per CU: each loads every charge record once per step
+        // for all the lines it holds
+        EM_TRY(allow_lds(k_trace_tail, kTailReserve));
         int per_cu = 0;
-        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_tail, kTailLanes, 0));
-        if (per_cu < 1) EM_FAIL(EM_ERR_INVALID, "second-phase workgroup does not fit a CU");
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * dev.cus, cfg.n);
-        hipLaunchKernelGGL(k_trace_tail, dim3(grid), dim3(kTailLanes), 0, stream, cfg, padded, pp, x,
-                           y, z, sp, kEndsCap, st, ctrl, surv);
+        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_tail, kTailLanes,
+                                                             kTailReserve));
+        if (per_cu != 2) EM_FAIL(EM_ERR_INVALID, "second-phase workgroups per CU: %d, not 2", per_cu);
+        hipLaunchKernelGGL(k_trace_tail, dim3(2 * dev.cus), dim3(kTailLanes), kTailReserve, stream,
+                           cfg, padded, pp, x, y, z, sp, kEndsCap, st, ctrl, surv);
         EM_HIP(hipGetLastError());
     }
     const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
