@@ -256,8 +256,9 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                        "block": launch_s.block},
             "lines_past_first_phase": int((st > 192).sum().item()),
             "note": "lines end at sinks; two phases: lane groups take a new line whenever theirs "
-                    "ends, up to 192 steps (launch = this phase's shape), then one 256-lane "
-                    "workgroup per line still moving; rows after a line's end are then filled"}
+                    "ends, up to 192 steps (launch = this phase's shape), then 256-lane "
+                    "workgroups (two per CU) trace the lines still moving, up to 8 at a time; rows "
+                    "after a line's end are then filled"}
         del rows
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
     if on("batch_512"):

@@ -67,8 +67,8 @@ def _device_key() -> str:
 # 1%) going to fewer lanes per line.  [S, block, tile, ms at 48 rows]
 SHIPPED = {
     # 16,384 charges, 65,536 lines, angle rule: all steps; with line ends the
-    # first phase of the two-phase trace (56.5 ms; 70 ms with one-wave blocks
-    # over 512-charge tiles, 76 ms with 8 lanes per line)
+    # first phase of the two-phase trace (51 ms in all; with 16 lanes per line
+    # 57 ms, with 8 lanes 76 ms, with one-wave blocks over 512-charge tiles 70 ms)
     "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, None],
     "plain:gfx950:256:16384:65536:2:1": [32, 512, 4096, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule

@@ -133,8 +133,9 @@ int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x,
  * needed by em_adjoint_dev_f32 every ckpt_every steps; size it with
  * em_ckpt_floats().  d_steps (may be NULL): n step counts.
  * The angle rule with stop and no checkpoints is traced in two phases (lane
- * groups that take a new line when theirs ends, then one workgroup per line
- * still moving after 192 steps); `launch` then sets the first phase's shape.
+ * groups that take a new line when theirs ends, then workgroups that trace the
+ * lines still moving after 192 steps, up to 8 at a time); `launch` then sets
+ * the first phase's shape.
  * That path keeps per-device work memory, allocated on first use and grown as
  * needed; under stream capture it must already be large enough (a call of
  * the same or a larger size made outside capture), else EM_ERR_INVALID. */
