@@ -23,6 +23,7 @@
 // one-kernel path; only the split of each charge sum over lanes differs, so
 // results agree with it to FP32 summation rounding.
 #include <mutex>
+#include <vector>
 
 #include "em_internal.hpp"
 
@@ -441,9 +442,12 @@ int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, cons
 
 // Per-device work memory of the two-phase trace: counters, step counts (when
 // the caller keeps none), the set-aside list and the padded charge copy.
-// Calls on one device use it in turn: each call's work waits for the previous
-// call's (an event), except under stream capture, where ordering is the
-// capturing caller's.
+// Eager calls on one device use it in turn: each call's work waits for the
+// previous eager call's (an event).  A captured graph uses the buffer current
+// at capture; its replays are ordered only by the caller's streams, so a
+// replay that overlaps another call on the same device races with it.
+// A buffer that is outgrown is retired, not freed: a graph captured earlier
+// still holds its address.  Retired buffers are freed by em_shutdown.
 struct EndsScratch {
     std::mutex mu;
     int device = -1;
@@ -451,6 +455,7 @@ struct EndsScratch {
     size_t bytes = 0;
     hipEvent_t done = nullptr;
     bool recorded = false;
+    std::vector<char*> retired;
 };
 EndsScratch g_ends[64];
 
@@ -483,12 +488,13 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
             EM_FAIL(EM_ERR_INVALID,
                     "line-ends work memory must grow to %zu bytes: make a first call of this "
                     "size outside stream capture", need);
-        if (w.recorded) EM_HIP(hipEventSynchronize(w.done));
-        if (w.buf) EM_HIP(hipFree(w.buf));
-        w.buf = nullptr;
-        w.bytes = 0;
-        EM_HIP(hipMalloc(reinterpret_cast<void**>(&w.buf), need));
-        w.bytes = need;
+        // at least doubling: the retired buffers total less than the live one
+        const size_t size = std::max(need, 2 * w.bytes);
+        char* grown = nullptr;
+        EM_HIP(hipMalloc(reinterpret_cast<void**>(&grown), size));
+        if (w.buf) w.retired.push_back(w.buf);
+        w.buf = grown;
+        w.bytes = size;
     }
     if (w.recorded && !capturing) EM_HIP(hipStreamWaitEvent(stream, w.done, 0));
 
@@ -535,6 +541,9 @@ int ends_release_all()
         if (hipSetDevice(w.device) != hipSuccess) rc = EM_ERR_HIP;
         if (w.recorded && hipEventSynchronize(w.done) != hipSuccess) rc = EM_ERR_HIP;
         if (w.buf && hipFree(w.buf) != hipSuccess) rc = EM_ERR_HIP;
+        for (char* r : w.retired)
+            if (hipFree(r) != hipSuccess) rc = EM_ERR_HIP;
+        w.retired.clear();
         if (w.done && hipEventDestroy(w.done) != hipSuccess) rc = EM_ERR_HIP;
         w.buf = nullptr;
         w.bytes = 0;

@@ -136,9 +136,13 @@ int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x,
  * groups that take a new line when theirs ends, then workgroups that trace the
  * lines still moving after 192 steps, up to 8 at a time); `launch` then sets
  * the first phase's shape.
- * That path keeps per-device work memory, allocated on first use and grown as
- * needed; under stream capture it must already be large enough (a call of
- * the same or a larger size made outside capture), else EM_ERR_INVALID. */
+ * That path keeps per-device work memory (about 32 bytes per line plus a
+ * padded copy of the charges), allocated on first use and grown by doubling;
+ * outgrown buffers stay allocated, since a graph captured earlier may use
+ * them, until em_shutdown.  Under stream capture it must already be large
+ * enough (a call of the same or a larger size made outside capture), else
+ * EM_ERR_INVALID.  Eager calls on one device are ordered among themselves; a
+ * graph replay that overlaps another call on the same device races with it. */
 int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y,
                      float* d_z, size_t n, size_t len, size_t pitch,
                      const em_step* step, const em_launch* launch,

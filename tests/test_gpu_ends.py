@@ -243,3 +243,32 @@ def test_graph_replays_match_eager(gpu):
         torch.cuda.synchronize()
         assert np.array_equal(rows.cpu().numpy(), eager_rows)
         assert np.array_equal(steps.cpu().numpy().astype(np.int64), eager_steps)
+
+
+def test_graph_replay_after_work_memory_grows(gpu):
+    """A graph captured at one size still replays correctly after an eager call
+    has grown the work memory past anything used before (the outgrown buffer
+    the graph holds is retired, not freed)."""
+    w = gen.lines_8k()
+    seeds = w.seeds[::16]
+    length = 250
+    n = seeds.shape[0]
+    charges = dev(w.charges)
+    eager_rows, eager_steps = ends_trace(gpu, w.charges, seeds, length)
+    rows = gpu.seed_rows(dev(seeds), length)
+    steps = torch.empty(n, dtype=torch.int32, device="cuda")
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        gpu.trace_into(charges, rows, n, rules.angle(), steps=steps)
+    # 250,000 lines: more work memory than any other test asks for
+    big_seeds = np.tile(w.seeds, (31, 1))[:250000]
+    big = gpu.seed_rows(dev(big_seeds), 3)
+    gpu.trace_into(charges, big, big_seeds.shape[0], rules.angle())
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(big).all())
+    for _ in range(2):
+        rows[:, 1:].fill_(-1.0)
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(rows.cpu().numpy(), eager_rows)
+        assert np.array_equal(steps.cpu().numpy().astype(np.int64), eager_steps)
