@@ -86,7 +86,10 @@ static int run_plan(em_batch* b, const float* d_charges, float* d_x, float* d_y,
     int dev = 0;
     EM_HIP(hipGetDevice(&dev));
     if (dev != b->device) EM_FAIL(EM_ERR_INVALID, "plan was created on device %d, current is %d", b->device, dev);
-    EM_TRY(set_ckpt_offsets(b, d_ckpt ? every : 0, stream));
+    // the descriptors' checkpoint offsets are read only when checkpoints are
+    // written: a plain run leaves them as the last checkpointed run set them
+    // (a graph captured with checkpoints keeps seeing its own offsets)
+    if (d_ckpt) EM_TRY(set_ckpt_offsets(b, every, stream));
     const float4* ch = reinterpret_cast<const float4*>(d_charges);
     const uint32_t ev = every ? every : 1;
     if (!b->use_graph)

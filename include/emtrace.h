@@ -201,7 +201,11 @@ int em_adjoint_dev_f32(const float* d_charges, size_t p, const float* d_x,
 
 /* Batched adjoint over the configurations of a plan (charges, rows and
  * gradients packed as for em_batch_run; checkpoints from
- * em_batch_run_ckpt; grad arrays indexed like charges / lines). */
+ * em_batch_run_ckpt; grad arrays indexed like charges / lines).
+ * The plan keeps device descriptors and work lists for one ckpt_every at a
+ * time: a call with another ckpt_every rebuilds them, so a graph that
+ * captured em_batch_run_ckpt / em_batch_adjoint of this plan must be captured
+ * again after such a call.  Plain em_batch_run calls change nothing. */
 size_t em_batch_ckpt_floats(const em_batch* plan, uint32_t ckpt_every);
 int em_batch_run_ckpt(em_batch* plan, const float* d_charges, float* d_x,
                       float* d_y, float* d_z, float* d_ckpt,

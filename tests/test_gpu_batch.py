@@ -113,3 +113,43 @@ def test_autotuner_picks_a_valid_shape(gpu, tmp_path, monkeypatch):
     assert torch.equal(a, b)
     blaunch, brep = autotune.tune_batch(_configs(count=6, seed=2, length=40), rules.angle(), rows=8)
     assert blaunch.lanes_per_line >= 1 and any(np.isfinite(r[3]) for r in brep)
+
+
+def test_captured_checkpointed_graph_survives_plain_runs(gpu):
+    """run_ckpt + adjoint captured into a graph; plain (checkpoint-free) eager
+    runs of the same plan between replays leave the captured descriptors
+    alone, so every replay equals the eager forward + adjoint."""
+    ws = _configs(count=6, seed=8, length=70)
+    layout, charges, lines = gpu.pack_batch(ws)
+    plan = gpu.BatchPlan(layout, rules.angle(), None, use_graph=False)
+    every = 10
+    ck = torch.empty(plan.ckpt_size(every), dtype=torch.float32, device="cuda")
+    G = torch.randn_like(lines)
+    grad = torch.empty_like(charges)
+    seeded = lines.clone()
+    plan.run_ckpt(charges, lines, ck, every)
+    plan.adjoint(charges, lines, ck, every, G, grad)
+    torch.cuda.synchronize()
+    want_lines, want_grad = lines.clone(), grad.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.run_ckpt(charges, lines, ck, every)
+        plan.adjoint(charges, lines, ck, every, G, grad)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run_ckpt(charges, lines, ck, every)
+        plan.adjoint(charges, lines, ck, every, G, grad)
+    for rep in range(3):
+        plain = seeded.clone()
+        plan.run(charges, plain)  # eager, no checkpoints
+        torch.cuda.synchronize()
+        lines.copy_(seeded)
+        grad.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(lines, want_lines), rep
+        assert torch.equal(grad, want_grad), rep
+        assert torch.equal(plain, want_lines), rep
