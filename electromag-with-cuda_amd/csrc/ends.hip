@@ -468,6 +468,12 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
                       hipStream_t stream)
 {
     if (dev.device < 0 || dev.device >= 64) EM_FAIL(EM_ERR_INVALID, "device %d", dev.device);
+    // the line counter passes n by up to one fetch per lane group, and the
+    // second phase's list positions by up to kSlots per workgroup: 32 bits hold both
+    if (cfg.n > 0xFFF00000u) EM_FAIL(EM_ERR_INVALID, "line ends: n %u above 2^32 - 2^20", cfg.n);
+    // the second phase addresses the padded charges through a buffer
+    // descriptor whose size field counts bytes in 32 bits
+    if (cfg.p > (1u << 28) - kTailPad) EM_FAIL(EM_ERR_INVALID, "line ends: p %u above 2^28 - 4096", cfg.p);
     const uint32_t pp = std::max<uint32_t>(kTailPad, round_up(cfg.p, kTailPad));
     const size_t off_steps = align256(sizeof(EndsCtrl));
     const size_t steps_bytes = steps ? 0 : align256((size_t)cfg.n * sizeof(uint32_t));
