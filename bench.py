@@ -357,9 +357,29 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
         cl, sl = torch.from_numpy(wl.charges).cuda(), torch.from_numpy(wl.seeds).cuda()
         rl = emtrace.seed_rows(sl, wl.len)
         _, kl = timed(lambda: emtrace.trace_into(cl, rl, wl.n, rules.ref(1.0)), 3, 1)
+        # the reference application's own call (ElectroMag.cpp:305): host arrays
+        # freshly allocated (pages not yet touched), through the drop-in entry
+        import numpy as np
+        calls = []
+        for _ in range(2):
+            hx, hy, hz = (np.empty(wl.len * wl.n, np.float32) for _ in range(3))
+            hx[:wl.n], hy[:wl.n], hz[:wl.n] = wl.seeds[:, 0], wl.seeds[:, 1], wl.seeds[:, 2]
+            t0 = time.perf_counter()
+            rc, perf = emtrace.calc_lines(wl.charges, hx, hy, hz, wl.n, 1.0, True)
+            calls.append((time.perf_counter() - t0, perf))
+            if rc != 0:
+                raise SystemExit(f"drop-in call failed: {rc}")
+            del hx, hy, hz
+        wall_d, perf_d = min(calls, key=lambda c: c[0])
         res["legacy_default_preset"] = {
             "interactions_per_s": float(wl.interactions) / min(kl), "kernel_ms": min(kl) * 1e3,
-            "charges": wl.p, "lines": wl.n, "rows": wl.len}
+            "charges": wl.p, "lines": wl.n, "rows": wl.len,
+            "dropin_host_arrays": {
+                "call_s": wall_d, "h2d_s": perf_d["h2d_s"], "kernel_s": perf_d["kernel_s"],
+                "d2h_s": perf_d["d2h_s"], "d2h_GB_per_s": perf_d["d2h_bytes"] / perf_d["d2h_s"] / 1e9,
+                "reference_gflops_formula": perf_d["gflops"],
+                "note": "em_calc_lines_f32 on fresh numpy arrays (best of 2 calls); the reference "
+                        "app's CalcField call, rows copied back into pageable memory"}}
         del rl, cl, sl
     # 7. particle step (Particle_System.cpp:108-190): 65,536 particles among the
     #    16,384 lines_64k charges, one step with the written-back state

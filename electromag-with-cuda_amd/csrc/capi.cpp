@@ -278,9 +278,15 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
         EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
                                    nullptr, 1, dsteps, b.stream));
     EM_HIP(hipEventRecord(b.ev[2], b.stream));
+    const size_t out_bytes = (len - 1) * row;
+    {
+        // while the GPU traces: fault in the caller's output rows
+        const CopySeg out[3] = {{x + n, nullptr, out_bytes}, {y + n, nullptr, out_bytes},
+                                {z + n, nullptr, out_bytes}};
+        prefault_pages(out, 3);
+    }
     EM_HIP(hipEventSynchronize(b.ev[2]));
     const double td = now_s();
-    const size_t out_bytes = (len - 1) * row;
     const CopySeg segs[4] = {{x + n, dx + n, out_bytes}, {y + n, dy + n, out_bytes},
                              {z + n, dz + n, out_bytes}, {steps, dsteps, n * sizeof(uint32_t)}};
     EM_TRY(d2h_staged(dev.device, segs, steps ? 4 : 3, b.stream));

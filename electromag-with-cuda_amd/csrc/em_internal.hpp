@@ -113,6 +113,9 @@ int d2h_staged(int device, const CopySeg* segs, int nseg, hipStream_t stream);
 // copies have completed
 int h2d_staged(int device, const CopySeg* segs, int nseg, hipStream_t stream);
 int staging_release_all();
+// Write one byte of every 4 KiB page of each segment's dst (output memory the
+// copy back will overwrite), with a small host thread team; a no-op below 16 MiB.
+void prefault_pages(const CopySeg* segs, int nseg);
 
 // adjoint (adjoint.hip)
 struct AdjPlan;

@@ -1,5 +1,10 @@
 // staging.cpp -- host <-> device copies for the host-array entry points.
 //
+// Output arrays handed in by a caller are often fresh allocations whose pages
+// the kernel has not touched yet: a copy into them takes a page fault per
+// 4 KiB.  prefault_pages() takes those faults with a thread team while the
+// GPU computes, so the copy back afterwards runs into resident pages.
+//
 // The host-array forms (the drop-in calc-lines calls, the FP64 and particle
 // host forms) hand rows back into the caller's own arrays, which are ordinary
 // pageable memory.  A plain hipMemcpy into pageable memory runs at a few GB/s.
@@ -155,6 +160,29 @@ int h2d_staged(int device, const CopySeg* segs, int nseg, hipStream_t stream)
     for (int i = 0; i < 2; ++i)
         if (used[i]) EM_HIP(hipEventSynchronize(st.ev[i]));
     return EM_OK;
+}
+
+void prefault_pages(const CopySeg* segs, int nseg)
+{
+    constexpr size_t kPage = 4096;
+    size_t total = 0;
+    for (int i = 0; i < nseg; ++i) total += segs[i].bytes;
+    if (total < (16ull << 20)) return;  // a few thousand faults: not worth a team
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned T = std::min(16u, std::max(1u, hw / 2));
+    auto touch = [&](unsigned t) {
+        for (int i = 0; i < nseg; ++i) {
+            char* base = static_cast<char*>(segs[i].dst);
+            const size_t per = (segs[i].bytes + T - 1) / T;
+            const size_t lo = std::min(segs[i].bytes, per * t), hi = std::min(segs[i].bytes, lo + per);
+            // one write per page; the copy back overwrites every byte later
+            for (size_t off = lo; off < hi; off += kPage) *reinterpret_cast<volatile char*>(base + off) = 0;
+        }
+    };
+    std::vector<std::thread> team;
+    for (unsigned t = 1; t < T; ++t) team.emplace_back(touch, t);
+    touch(0);
+    for (auto& th : team) th.join();
 }
 
 int staging_release_all()
