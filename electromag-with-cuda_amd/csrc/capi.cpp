@@ -101,7 +101,10 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
         int S = 1;
         while (S < 32 && p / (2ull * S) >= 64) S *= 2;
         s.S = (req && req->lanes_per_line) ? req->lanes_per_line : S;
-        s.block = (req && req->block) ? req->block : 512;
+        // blocks of up to 512 threads, small enough that the grid reaches every CU
+        int block = 512;
+        while (block > 64 && n * (uint64_t)s.S / (uint64_t)block < (uint64_t)dev.cus) block /= 2;
+        s.block = (req && req->block) ? req->block : block;
         s.tile = (req && req->tile) ? (uint32_t)req->tile : tile_for(dev, p, s.S, 2);
         EM_TRY(validate_shape(dev, s));
         out = s;
@@ -238,8 +241,9 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    const bool ends = st->stop != 0;  // no checkpoints here: the two-phase line-ends trace
-    EM_TRY(choose_shape(dev, p, n, launch, sh, ends, ends));
+    // line ends (no checkpoints here) over many charges: the two-phase trace
+    const bool ends = st->stop != 0 && p >= kEndsMinCharges;
+    EM_TRY(choose_shape(dev, p, n, launch, sh, st->stop != 0, ends));
 
     std::lock_guard<std::mutex> lk(g_mu);
     HostPathBuffers& b = g_hp[dev.device];
@@ -405,9 +409,10 @@ int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, f
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    // line ends without checkpoints: the two-phase trace; with checkpoints
-    // the one-kernel path, whose steps the adjoint's recompute repeats
-    const bool ends = step->stop && !d_ckpt;
+    // line ends without checkpoints over many charges: the two-phase trace;
+    // with checkpoints the one-kernel path, whose steps the adjoint's
+    // recompute repeats
+    const bool ends = step->stop && !d_ckpt && p >= kEndsMinCharges;
     EM_TRY(choose_shape(dev, p, n, launch, sh, step->stop != 0, ends));
     Cfg cfg{};
     cfg.pitch = pitch;

@@ -88,6 +88,11 @@ int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const It
                             float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
                             uint32_t* steps, hipStream_t stream);
 int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid);
+// Problems with at least this many charges trace the angle rule with line
+// ends (and no checkpoints) through launch_trace_ends; smaller ones, whose
+// steps are short sums, through the one-kernel path.
+constexpr uint32_t kEndsMinCharges = 4096;
+
 // The angle rule with line ends and no checkpoints, in two phases (ends.hip):
 // lane groups that take a new line whenever theirs ends, up to a step cap;
 // then one 256-lane workgroup per line still moving; then a coalesced fill of

@@ -19,9 +19,12 @@
 //   3. k_ends_fill: rows after a line's end get its end point (the reference
 //      convention of the one-kernel path: an ended line keeps its last point),
 //      written row-major and coalesced.
-// Every line takes the same steps with the same arithmetic as in the
-// one-kernel path; only the split of each charge sum over lanes differs, so
-// results agree with it to FP32 summation rounding.
+// The second phase runs for problems of at least 4,096 charges, where a
+// line's step is a long sum; below that the first phase takes every line to
+// its end.  Every line takes the same steps with the same arithmetic as in
+// the one-kernel path; only the split of each charge sum over lanes differs
+// (and not at all below the cap or the 4,096-charge threshold), so results
+// agree with it to FP32 summation rounding.
 #include <mutex>
 #include <vector>
 
@@ -35,14 +38,15 @@ constexpr uint32_t kEndsCap = 192;     // steps a line may take in the first pha
 constexpr int kTailLanes = 256;        // lanes of a second-phase workgroup
 constexpr int kTailWaves = kTailLanes / 64;
 constexpr int kSlots = 8;              // lines a second-phase workgroup traces at once
-constexpr uint32_t kTailPad = kTailLanes * kGroup;  // granule of the padded charge copy
+constexpr uint32_t kTailMinCharges = kEndsMinCharges;  // 4,096 = kTailLanes * kGroup
+static_assert(kTailMinCharges == kTailLanes * kGroup, "the second phase's bulk needs 2 groups");
 constexpr size_t kTailReserve = 64 * 1024;  // LDS that keeps two second-phase workgroups per CU
 constexpr uint32_t kFillRows = 64;     // rows per fill thread
 
 struct EndsCtrl {
     uint32_t next_line;  // first phase: next line to hand out
     uint32_t nsurv;      // lines set aside at the cap
-    uint32_t unused0;
+    uint32_t next_resume;  // continuation: next set-aside line to hand out
     uint32_t unused;
 };
 
@@ -55,24 +59,31 @@ __global__ void k_ends_prepare(EndsCtrl* __restrict__ ctrl, const float4* __rest
     if (i == 0) {
         ctrl->next_line = 0u;
         ctrl->nsurv = 0u;
-        ctrl->unused0 = 0u;
+        ctrl->next_resume = 0u;
         ctrl->unused = 0u;
     }
     if (i < pp) padded[i] = i < p ? charges[i] : make_float4(kFar, kFar, kFar, 0.f);
 }
 
-// First phase.  `surv` receives two float4 per set-aside line:
+// First phase (RESUME = false): lines 0..n-1 from their seeds; a line still
+// moving after `cap` steps is set aside in `surv`, two float4 per line:
 // {x, y, z, h} and {ux, uy, uz, line index as bits}.
-template <int S>
+// Continuation (RESUME = true): when more lines were set aside than the second
+// phase holds at once (more than `tail_max`), this kernel takes them instead,
+// from step `start` + 1 to the end, with the same refill; otherwise it exits
+// at once (the second phase has them).
+template <int S, bool RESUME>
 __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __restrict__ charges,
                                                     float* __restrict__ X, float* __restrict__ Y,
                                                     float* __restrict__ Z, StepP sp,
-                                                    uint32_t tile_cap, uint32_t cap,
-                                                    uint32_t* __restrict__ steps,
+                                                    uint32_t tile_cap, uint32_t cap, uint32_t start,
+                                                    uint32_t tail_max, uint32_t* __restrict__ steps,
                                                     EndsCtrl* __restrict__ ctrl,
                                                     float4* __restrict__ surv)
 {
     extern __shared__ float4 tile[];
+    const uint32_t nsurv = RESUME ? __builtin_amdgcn_readfirstlane(ctrl->nsurv) : 0u;
+    if (RESUME && nsurv <= tail_max) return;  // uniform: before any barrier
     const int k = threadIdx.x % S;
     const int leader = (int)(threadIdx.x % 64) - k;  // wave lane of the group's lane 0
     const uint32_t p = cfg.p;
@@ -95,9 +106,20 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
         if (!active && !drained) {
             // the whole group is here (the branch is uniform over it)
             uint32_t got = 0;
-            if (k == 0) got = atomicAdd(&ctrl->next_line, 1u);
+            if (k == 0) got = atomicAdd(RESUME ? &ctrl->next_resume : &ctrl->next_line, 1u);
             got = __shfl(got, leader, 64);
-            if (got < cfg.n) {
+            if (RESUME && got < nsurv) {
+                const float4 a = surv[2ull * got];
+                const float4 b = surv[2ull * got + 1];
+                line = __float_as_uint(b.w);
+                st.px = a.x; st.py = a.y; st.pz = a.z; st.h = a.w;
+                st.ux = b.x; st.uy = b.y; st.uz = b.z;
+                st.Px = st.Py = st.Pz = 0.f;
+                st.alive = true;
+                st.steps = start;
+                s = start + 1;
+                active = true;
+            } else if (!RESUME && got < cfg.n) {
                 line = got;
                 const uint64_t b = cfg.line_off + line;
                 st.px = X[b];
@@ -134,7 +156,7 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
             if (!st.alive || s + 1 >= cfg.len) {
                 if (k == 0) steps[cfg.line_index + line] = st.steps;
                 active = false;
-            } else if (s == cap) {
+            } else if (!RESUME && s == cap) {
                 if (k == 0) {
                     const uint32_t i = atomicAdd(&ctrl->nsurv, 1u);
                     surv[2ull * i] = make_float4(st.px, st.py, st.pz, st.h);
@@ -160,10 +182,12 @@ __device__ __forceinline__ float4 buffer_record(__amdgpu_buffer_rsrc_t rsrc, uin
 }
 
 // Coulomb sums at the NACT points of a workgroup's active slots over pp padded
-// records (a multiple of kTailPad) split over the kTailLanes lanes: every
-// record a lane loads serves all NACT points.  The software pipeline of
-// pipelined_tile (two register sets of kUnroll records, one set's loads in
-// flight while the other is consumed), reading global memory.  Each lane
+// records (a multiple of kTailLanes, at least kTailMinCharges) split over the
+// kTailLanes lanes: every record a lane loads serves all NACT points.  The
+// bulk runs the software pipeline of pipelined_tile (two register sets of
+// kUnroll records, one set's loads in flight while the other is consumed),
+// reading global memory; the last pp mod (2 kUnroll kTailLanes) records
+// follow one per lane at a time.  Each lane
 // sums pp / kTailLanes records per point in plain FP32 (the blocked
 // compensation of the other kernels folds every 128 records, so it would
 // change nothing here); the lanes' sums are then combined by a butterfly and
@@ -174,7 +198,7 @@ __device__ __forceinline__ void tail_sums(__amdgpu_buffer_rsrc_t rsrc, uint32_t 
                                           const float4* __restrict__ pts, float4* __restrict__ part)
 {
     constexpr uint32_t kStride = kTailLanes * kUnroll;  // records per group
-    const uint32_t ngroups = pp / kStride;  // even, >= 2
+    const uint32_t ngroups = (pp / kStride) & ~1u;  // even, >= 2 since pp >= kTailMinCharges
     const uint32_t lane_off = (uint32_t)tid * 16u;
     float px[NACT], py[NACT], pz[NACT], ex[NACT], ey[NACT], ez[NACT];
 #pragma unroll
@@ -205,6 +229,11 @@ __device__ __forceinline__ void tail_sums(__amdgpu_buffer_rsrc_t rsrc, uint32_t 
 #pragma unroll
             for (int l = 0; l < NACT; ++l) coulomb(b[u], px[l], py[l], pz[l], ex[l], ey[l], ez[l]);
         __builtin_amdgcn_sched_barrier(0);
+    }
+    for (uint32_t r = ngroups * kStride; r < pp; r += kTailLanes) {
+        const float4 c = buffer_record(rsrc, lane_off, r);
+#pragma unroll
+        for (int l = 0; l < NACT; ++l) coulomb(c, px[l], py[l], pz[l], ex[l], ey[l], ez[l]);
     }
     const int wave = tid / 64;
 #pragma unroll
@@ -298,12 +327,12 @@ __device__ __forceinline__ uint32_t refill_slots(Slot& sl, bool done, uint32_t n
 // other threads re-enter without that thread, which then never publishes the
 // next item.  Loop exits on LDS values go through readfirstlane, so they are
 // scalar branches.
-// `padded` holds pp records (a multiple of kTailPad), charges then zero-charge
-// padding; lane t sums records t, t + 256, t + 512, ...
+// `padded` holds pp records (a multiple of kTailLanes, >= kTailMinCharges),
+// charges then zero-charge padding; lane t sums records t, t + 256, t + 512, ...
 __global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4* __restrict__ padded,
                                                            uint32_t pp, float* __restrict__ X,
                                                            float* __restrict__ Y, float* __restrict__ Z,
-                                                           StepP sp, uint32_t cap,
+                                                           StepP sp, uint32_t cap, uint32_t tail_max,
                                                            uint32_t* __restrict__ steps,
                                                            const EndsCtrl* __restrict__ ctrl,
                                                            const float4* __restrict__ surv)
@@ -314,7 +343,10 @@ __global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4
     __shared__ uint32_t s_nact;
     const int tid = threadIdx.x;
     const int wave = tid / 64;
-    const uint32_t nsurv = ctrl->nsurv;  // final: the first phase has completed
+    // final: the first phase has completed; beyond tail_max lines the
+    // continuation of the first phase takes them instead
+    const uint32_t nsurv = __builtin_amdgcn_readfirstlane(ctrl->nsurv);
+    if (nsurv > tail_max) return;  // uniform: before any barrier
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float4*>(padded), (short)0, (int)(pp * sizeof(float4)), 0x00020000);
     Slot sl;  // meaningful in lanes 0..kSlots-1 of wave 0
@@ -405,39 +437,48 @@ __global__ void k_ends_fill(Cfg cfg, float* __restrict__ X, float* __restrict__ 
     }
 }
 
-template <int S>
+template <int S, bool RESUME>
 int launch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
-                  float* x, float* y, float* z, const StepP& sp, uint32_t* steps, EndsCtrl* ctrl,
-                  float4* surv, hipStream_t stream)
+                  float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t start,
+                  uint32_t tail_max, uint32_t* steps, EndsCtrl* ctrl, float4* surv,
+                  hipStream_t stream)
 {
     const size_t lds = (size_t)sh.tile * sizeof(float4);
-    EM_TRY(allow_lds(k_trace_ends<S>, lds));
+    EM_TRY(allow_lds(k_trace_ends<S, RESUME>, lds));
     int per_cu = 0;
-    EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_ends<S>, sh.block, lds));
+    EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_ends<S, RESUME>, sh.block,
+                                                         lds));
     if (per_cu < 1)
         EM_FAIL(EM_ERR_INVALID, "launch shape does not fit a CU (block %d, tile %u)", sh.block, sh.tile);
     const uint64_t groups_per_block = (uint64_t)sh.block / S;
     const uint64_t want = (cfg.n + groups_per_block - 1) / groups_per_block;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * dev.cus, want);
-    hipLaunchKernelGGL(k_trace_ends<S>, dim3(grid), dim3(sh.block), lds, stream, cfg, charges, x, y,
-                       z, sp, sh.tile, kEndsCap, steps, ctrl, surv);
+    hipLaunchKernelGGL((k_trace_ends<S, RESUME>), dim3(grid), dim3(sh.block), lds, stream, cfg,
+                       charges, x, y, z, sp, sh.tile, cap, start, tail_max, steps, ctrl, surv);
     EM_HIP(hipGetLastError());
     return EM_OK;
 }
 
+template <bool RESUME>
 int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
-                    float* x, float* y, float* z, const StepP& sp, uint32_t* steps, EndsCtrl* ctrl,
-                    float4* surv, hipStream_t stream)
+                    float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t start,
+                    uint32_t tail_max, uint32_t* steps, EndsCtrl* ctrl, float4* surv,
+                    hipStream_t stream)
 {
+#define EM_PHASE1(SS)                                                                            \
+    case SS:                                                                                     \
+        return launch_phase1<SS, RESUME>(sh, dev, cfg, charges, x, y, z, sp, cap, start, tail_max, \
+                                         steps, ctrl, surv, stream)
     switch (sh.S) {
-        case 1: return launch_phase1<1>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
-        case 2: return launch_phase1<2>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
-        case 4: return launch_phase1<4>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
-        case 8: return launch_phase1<8>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
-        case 16: return launch_phase1<16>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
-        case 32: return launch_phase1<32>(sh, dev, cfg, charges, x, y, z, sp, steps, ctrl, surv, stream);
+        EM_PHASE1(1);
+        EM_PHASE1(2);
+        EM_PHASE1(4);
+        EM_PHASE1(8);
+        EM_PHASE1(16);
+        EM_PHASE1(32);
         default: EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1,2,4,8,16,32}", sh.S);
     }
+#undef EM_PHASE1
 }
 
 // Per-device work memory of the two-phase trace: counters, step counts (when
@@ -473,17 +514,23 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
     if (cfg.n > 0xFFF00000u) EM_FAIL(EM_ERR_INVALID, "line ends: n %u above 2^32 - 2^20", cfg.n);
     // the second phase addresses the padded charges through a buffer
     // descriptor whose size field counts bytes in 32 bits
-    if (cfg.p > (1u << 28) - kTailPad) EM_FAIL(EM_ERR_INVALID, "line ends: p %u above 2^28 - 4096", cfg.p);
-    const uint32_t pp = std::max<uint32_t>(kTailPad, round_up(cfg.p, kTailPad));
+    if (cfg.p > (1u << 28) - kTailLanes) EM_FAIL(EM_ERR_INVALID, "line ends: p %u above 2^28 - 256", cfg.p);
+    // The second phase pays off when a line's step is a long sum.  Below
+    // kTailMinCharges charges the first phase's wide groups take every line
+    // to its end (a step is then at most 128 records per lane), and no
+    // padded copy is made.
+    const bool tail = cfg.p >= kTailMinCharges && cfg.len > kEndsCap + 1;
+    const uint32_t cap = tail ? kEndsCap : cfg.len;
+    const uint32_t pp = tail ? round_up(cfg.p, kTailLanes) : 0u;
     const size_t off_steps = align256(sizeof(EndsCtrl));
     const size_t steps_bytes = steps ? 0 : align256((size_t)cfg.n * sizeof(uint32_t));
     const size_t off_surv = off_steps + steps_bytes;
     const size_t off_pad = off_surv + align256((size_t)cfg.n * 2 * sizeof(float4));
     const size_t need = off_pad + (size_t)pp * sizeof(float4);
 
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    EM_HIP(hipStreamIsCapturing(stream, &cap));
-    const bool capturing = cap != hipStreamCaptureStatusNone;
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    EM_HIP(hipStreamIsCapturing(stream, &capture));
+    const bool capturing = capture != hipStreamCaptureStatusNone;
 
     EndsScratch& w = g_ends[dev.device];
     std::lock_guard<std::mutex> lk(w.mu);
@@ -509,11 +556,14 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
     float4* surv = reinterpret_cast<float4*>(w.buf + off_surv);
     float4* padded = reinterpret_cast<float4*>(w.buf + off_pad);
 
-    hipLaunchKernelGGL(k_ends_prepare, dim3((pp + 255) / 256), dim3(256), 0, stream, ctrl, charges,
-                       cfg.p, padded, pp);
+    hipLaunchKernelGGL(k_ends_prepare, dim3(std::max<uint32_t>(1u, (pp + 255) / 256)), dim3(256), 0,
+                       stream, ctrl, charges, cfg.p, padded, pp);
     EM_HIP(hipGetLastError());
-    EM_TRY(dispatch_phase1(sh, dev, cfg, charges, x, y, z, sp, st, ctrl, surv, stream));
-    if (cfg.len > kEndsCap + 1) {
+    // the second phase holds 8 lines in each of its 2-per-CU workgroups at once
+    const uint32_t tail_max = 2u * (uint32_t)dev.cus * (uint32_t)kSlots;
+    EM_TRY(dispatch_phase1<false>(sh, dev, cfg, charges, x, y, z, sp, cap, 0u, tail_max, st, ctrl,
+                                  surv, stream));
+    if (tail) {
         // two workgroups per CU: each loads every charge record once per step
         // for all the lines it holds
         EM_TRY(allow_lds(k_trace_tail, kTailReserve));
@@ -522,8 +572,11 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
                                                              kTailReserve));
         if (per_cu != 2) EM_FAIL(EM_ERR_INVALID, "second-phase workgroups per CU: %d, not 2", per_cu);
         hipLaunchKernelGGL(k_trace_tail, dim3(2 * dev.cus), dim3(kTailLanes), kTailReserve, stream,
-                           cfg, padded, pp, x, y, z, sp, kEndsCap, st, ctrl, surv);
+                           cfg, padded, pp, x, y, z, sp, kEndsCap, tail_max, st, ctrl, surv);
         EM_HIP(hipGetLastError());
+        // more set-aside lines than that: the first phase's groups continue them
+        EM_TRY(dispatch_phase1<true>(sh, dev, cfg, charges, x, y, z, sp, cfg.len, kEndsCap,
+                                     tail_max, st, ctrl, surv, stream));
     }
     const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
     if (fill_grid.y > 65535u) EM_FAIL(EM_ERR_INVALID, "len %u exceeds the fill grid", cfg.len);

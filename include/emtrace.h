@@ -132,10 +132,11 @@ int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x,
  * d_ckpt (may be NULL): when non-NULL, the forward also stores the state
  * needed by em_adjoint_dev_f32 every ckpt_every steps; size it with
  * em_ckpt_floats().  d_steps (may be NULL): n step counts.
- * The angle rule with stop and no checkpoints is traced in two phases (lane
- * groups that take a new line when theirs ends, then workgroups that trace the
- * lines still moving after 192 steps, up to 8 at a time); `launch` then sets
- * the first phase's shape.
+ * The angle rule with stop, no checkpoints and p >= 4096 is traced in two
+ * phases (lane groups that take a new line when theirs ends, then workgroups
+ * that trace the lines still moving after 192 steps, up to 8 at a time, or the
+ * lane groups again when more than that many lines are still moving);
+ * `launch` then sets the first phase's shape.
  * That path keeps per-device work memory (about 32 bytes per line plus a
  * padded copy of the charges), allocated on first use and grown by doubling;
  * outgrown buffers stay allocated, since a graph captured earlier may use

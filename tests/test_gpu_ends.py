@@ -1,7 +1,11 @@
 """The two-phase trace of the angle rule with line ends (stop on, no
-checkpoints): lane groups that take a new line whenever theirs ends, up to a
-cap of CAP steps; then one 256-lane workgroup per line still moving; then the
-rows after each line's end are filled with its end point.
+checkpoints, at least TAIL_MIN charges): lane groups that take a new line
+whenever theirs ends, up to a cap of CAP steps; then 256-lane workgroups (two
+per CU) that trace the lines still moving, up to 8 at a time -- or, when more
+lines than that (TAIL_SLOTS on a 256-CU MI355X) reach the cap, the first
+phase's groups continue them; then the rows after each line's end are filled
+with its end point.  With fewer charges the same calls take the one-kernel
+path (the tests below cover both sides).
 
 Parity: against the CPU path (identical step counts, points within 1e-4 on
 well-conditioned lines, path distance to FP64 no worse than the CPU float32
@@ -17,11 +21,13 @@ import pytest
 import torch
 
 from emtrace import gen, rules
-from helpers import rows64_to_soa
-from test_gpu_trace import check_against_cpu, dev
+from helpers import parity_report, rows64_to_soa
+from test_gpu_trace import dev
 
 pytestmark = pytest.mark.gpu
 CAP = 192  # steps of the first phase (the second phase starts at step CAP + 1)
+TAIL_MIN = 4096  # fewer charges: the one-kernel path
+TAIL_SLOTS = 2 * 256 * 8  # lines the second phase holds at once on 256 CUs
 
 
 def escaping(p, n, half_side, length, seed):
@@ -34,12 +40,31 @@ def escaping(p, n, half_side, length, seed):
 
 
 def check_lines(orc, g_rows, c_rows, charges, seeds, length, tag, rule=None):
-    """The angle rule's bar (as for the one-kernel path): path distance to the
-    CPU path on lines whose CPU float32 path follows FP64, and distance to the
-    FP64 path no worse than the CPU float32 path's."""
+    """The angle rule's bar of the one-kernel path's tests, scaled for lines
+    that travel far: escaping lines reach |x| ~ 100, where one float32 ulp is
+    8e-6 and the CPU float32 path itself leaves the FP64 path, so fewer lines
+    qualify as well-conditioned (CPU float32 within 2e-5 of FP64, path
+    distance) -- at least 15 % here -- and on those, 99 % of GPU lines lie
+    within 1e-4 * max(1, R / 10) of the CPU path, R the line's largest
+    coordinate (1e-4 for lines within 10 of the origin).  Over all lines the
+    GPU's distance to FP64 is no worse than twice the CPU float32 path's
+    (median and 90th percentile)."""
     rule = rule or rules.angle()
     r64 = rows64_to_soa(orc.trace64(charges, seeds, length, rule))
-    check_against_cpu(f"ends {tag}", g_rows, c_rows, r64, charges, rule)
+    _, (_, path_g), _, _ = parity_report(g_rows, c_rows, charges, 1e-4)
+    _, (_, path_c64), _, _ = parity_report(c_rows, r64, charges, 1e-4)
+    _, (_, path_g64), _, _ = parity_report(g_rows, r64, charges, 1e-4)
+    cond = path_c64 <= 2e-5
+    print(f"[ends {tag}] conditioned {cond.sum()}/{len(cond)}; gpu-vs-cpu on them max "
+          f"{path_g[cond].max() if cond.any() else 0:.2e}; to FP64 median gpu "
+          f"{np.median(path_g64):.2e} cpu {np.median(path_c64):.2e}", flush=True)
+    assert np.isfinite(g_rows).all()
+    assert cond.mean() >= 0.15
+    R = np.nanmax(np.abs(c_rows), axis=(0, 1))
+    tol = 1e-4 * np.maximum(1.0, R / 10.0)
+    assert (path_g[cond] <= tol[cond]).mean() >= 0.99
+    assert np.median(path_g64) <= 2.0 * np.median(path_c64) + 1e-6
+    assert np.quantile(path_g64, 0.9) <= 2.0 * np.quantile(path_c64, 0.9) + 1e-6
 
 
 def same_steps(g_steps, c_steps):
@@ -70,37 +95,52 @@ def one_kernel_trace(gpu, charges, seeds, length, launch):
     return rows.cpu().numpy(), steps.cpu().numpy().astype(np.int64)
 
 
-def test_lines_8k_across_the_cap_vs_cpu_path(gpu, orc):
-    """lines_8k charges, 1,024 of its lines, 600 rows: most lines end in the
+def lines_64k_subset(step=64):
+    """lines_64k's 16,384 charges and every step-th of its seeds."""
+    w = gen.lines_64k()
+    return w.charges, w.seeds[::step]
+
+
+def test_across_the_cap_vs_cpu_path(gpu, orc):
+    """lines_64k charges, 1,024 of its lines, 600 rows: most lines end in the
     first phase, the escaping ones finish in the second."""
-    w = gen.lines_8k()
-    seeds = w.seeds[::8]
+    charges, seeds = lines_64k_subset()
+    w = gen.Workload(charges, seeds, 600)
     length = 600
     g_rows, g_steps = ends_trace(gpu, w.charges, seeds, length)
     rc, c_rows, c_steps = orc.calc_lines(w.charges, seeds, length, rules.angle(), with_steps=True)
     assert rc == 0
     past = (g_steps > CAP).sum()
-    print(f"[ends] lines_8k/600: lines past the cap {past}/{len(g_steps)}, "
+    print(f"[ends] lines_64k subset/600: lines past the cap {past}/{len(g_steps)}, "
           f"identical steps {(g_steps == c_steps).sum()}", flush=True)
     assert past >= 16  # the second phase ran
     assert (g_steps == c_steps.astype(np.int64)).mean() >= 0.99
-    r64 = rows64_to_soa(orc.trace64(w.charges, seeds, length, rules.angle()))
-    check_against_cpu("ends lines_8k/600", g_rows, c_rows, r64, w.charges, rules.angle())
+    check_lines(orc, g_rows, c_rows, w.charges, seeds, length, "lines_64k subset/600")
     # rows after a line's end repeat its end point
     for l in range(len(g_steps)):
         e = g_steps[l]
         assert (g_rows[:, e + 1:, l] == g_rows[:, e:e + 1, l]).all()
 
 
-def test_matches_one_kernel_path_bitwise_up_to_the_cap(gpu):
-    """Same launch shape: rows 0..CAP and every step count below the cap are
-    bit-identical to the one-kernel path; later rows agree closely."""
-    w = gen.lines_8k()
-    seeds = w.seeds[::8]
+@pytest.mark.parametrize("source", ["lines_8k", "lines_64k"])
+def test_matches_one_kernel_path_bitwise(gpu, source):
+    """Same launch shape as the one-kernel path (the checkpointed forward):
+    with fewer than TAIL_MIN charges (lines_8k, 1,000) both calls take the
+    one-kernel path and every row is bit-identical; with more (lines_64k,
+    16,384) rows 0..CAP and every line that ends before the cap are, and later
+    rows agree closely."""
+    if source == "lines_8k":
+        w = gen.lines_8k()
+        charges, seeds = w.charges, w.seeds[::8]
+    else:
+        charges, seeds = lines_64k_subset()
     length = 400
     launch = gpu.make_launch(32, 512, 512)
-    a_rows, a_steps = ends_trace(gpu, w.charges, seeds, length, launch)
-    b_rows, b_steps = one_kernel_trace(gpu, w.charges, seeds, length, launch)
+    a_rows, a_steps = ends_trace(gpu, charges, seeds, length, launch)
+    b_rows, b_steps = one_kernel_trace(gpu, charges, seeds, length, launch)
+    if charges.shape[0] < TAIL_MIN:
+        assert np.array_equal(a_steps, b_steps) and np.array_equal(a_rows, b_rows)
+        return
     short = b_steps < CAP
     assert np.array_equal(a_steps[short], b_steps[short])
     assert np.array_equal(a_rows[:, :CAP + 1], b_rows[:, :CAP + 1])
@@ -112,10 +152,9 @@ def test_matches_one_kernel_path_bitwise_up_to_the_cap(gpu):
 
 
 def test_deterministic_under_line_refill(gpu):
-    w = gen.lines_8k()
-    seeds = w.seeds[::4]
-    a = ends_trace(gpu, w.charges, seeds, 500)
-    b = ends_trace(gpu, w.charges, seeds, 500)
+    charges, seeds = lines_64k_subset(32)
+    a = ends_trace(gpu, charges, seeds, 500)
+    b = ends_trace(gpu, charges, seeds, 500)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
@@ -124,10 +163,12 @@ def test_deterministic_under_line_refill(gpu):
 SHORT_STEPS = rules.angle(hmax=0.05)
 
 
-@pytest.mark.parametrize("p", [1, 40, 4095, 4096, 4097, 12000])
+@pytest.mark.parametrize("p", [1, 40, 4095, 4096, 4097, 4352, 12000])
 def test_charge_counts_vs_cpu_path(gpu, orc, p):
-    """Charge counts around the second phase's padding granule (4,096) and
-    beyond one LDS tile (12,000: the first phase re-stages the charges)."""
+    """Charge counts on both sides of the second phase's threshold (4,096),
+    the 256-record granule of its charge copy (4,097: one record past it,
+    4,352 = 4,096 + 256) and beyond one LDS tile (12,000: the first phase
+    re-stages the charges)."""
     length = 260
     # the box grows with p: about 40 charges per 1,000 unit^3, as at p = 40
     ch, seeds = escaping(p, 96, 5.0 * max(1.0, p / 40.0) ** (1.0 / 3.0), length, seed=31 + p)
@@ -145,22 +186,25 @@ def test_charge_counts_vs_cpu_path(gpu, orc, p):
         check_lines(orc, g_rows, c_rows, ch, seeds, length, f"p={p}", SHORT_STEPS)
 
 
-def test_more_lines_set_aside_than_second_phase_slots(gpu, orc):
-    """3,000 escaping lines all reach the cap: the second phase (a few
-    workgroups per CU) takes them in several rounds."""
+@pytest.mark.parametrize("lines", [3000, 5000], ids=["second-phase-refills", "continuation"])
+def test_many_lines_past_the_cap(gpu, orc, lines):
+    """Escaping lines among 4,200 charges nearly all reach the cap.  3,000:
+    the second phase's workgroups (8 slots each) refill their slots; 5,000
+    (more than TAIL_SLOTS): the first phase's groups continue them instead."""
     length = CAP + 40
-    ch, seeds = escaping(64, 3000, 5.0, length, seed=5)
+    ch, seeds = escaping(4200, lines, 5.0 * (4200 / 40.0) ** (1.0 / 3.0), length, seed=5)
     g_rows, g_steps = ends_trace(gpu, ch, seeds, length, rule=SHORT_STEPS)
     rc, c_rows, c_steps = orc.calc_lines(ch, seeds, length, SHORT_STEPS, with_steps=True)
     assert rc == 0
     assert same_steps(g_steps, c_steps)
-    assert (g_steps == length - 1).sum() >= 2000  # more than the second phase holds at once
-    check_lines(orc, g_rows, c_rows, ch, seeds, length, "3000 escaping", SHORT_STEPS)
+    past = (g_steps > CAP).sum()
+    assert past >= (2000 if lines == 3000 else TAIL_SLOTS + 1)
+    check_lines(orc, g_rows, c_rows, ch, seeds, length, f"{lines} escaping", SHORT_STEPS)
 
 
 @pytest.mark.parametrize("length", [2, CAP, CAP + 1, CAP + 2])
 def test_lengths_at_the_cap(gpu, orc, length):
-    ch, seeds = escaping(50, 130, 5.0, length, seed=7)
+    ch, seeds = escaping(4500, 130, 5.0 * (4500 / 40.0) ** (1.0 / 3.0), length, seed=7)
     ch[::3, 3] *= -1.0  # some sinks: a mix of ending and escaping lines
     g_rows, g_steps = ends_trace(gpu, ch, seeds, length)
     rc, c_rows, c_steps = orc.calc_lines(ch, seeds, length, rules.angle(), with_steps=True)
@@ -188,7 +232,7 @@ def test_no_charges(gpu, orc):
 
 def test_pitch_wider_than_n(gpu):
     """Rows with pitch > n: columns n..pitch-1 are not written."""
-    w = gen.lines_8k()
+    w = gen.lines_64k()
     seeds = w.seeds[:200]
     length = 300
     ref_rows, ref_steps = ends_trace(gpu, w.charges, seeds, length)
@@ -205,7 +249,7 @@ def test_pitch_wider_than_n(gpu):
 
 def test_work_memory_reused_across_sizes(gpu):
     """A small call, a larger one (the work memory grows), the small one again."""
-    w = gen.lines_8k()
+    w = gen.lines_64k()
     small = ends_trace(gpu, w.charges, w.seeds[:100], 250)
     big = ends_trace(gpu, w.charges, w.seeds[:4000], 250)
     again = ends_trace(gpu, w.charges, w.seeds[:100], 250)
@@ -217,12 +261,12 @@ def test_graph_replays_match_eager(gpu):
     """Captured into a graph (the counters are cleared by a kernel, not a
     memset), replayed three times with eager work in between: every replay
     equals the eager result."""
-    w = gen.lines_8k()
-    seeds = w.seeds[::8]
+    charges_np, seeds = lines_64k_subset()
     length = 300
     n = seeds.shape[0]
-    charges = dev(w.charges)
-    eager_rows, eager_steps = ends_trace(gpu, w.charges, seeds, length)  # sizes the work memory
+    charges = dev(charges_np)
+    eager_rows, eager_steps = ends_trace(gpu, charges_np, seeds, length)  # sizes the work memory
+    w = gen.lines_8k()
     rows = gpu.seed_rows(dev(seeds), length)
     steps = torch.empty(n, dtype=torch.int32, device="cuda")
     s = torch.cuda.Stream()
@@ -249,12 +293,12 @@ def test_graph_replay_after_work_memory_grows(gpu):
     """A graph captured at one size still replays correctly after an eager call
     has grown the work memory past anything used before (the outgrown buffer
     the graph holds is retired, not freed)."""
-    w = gen.lines_8k()
-    seeds = w.seeds[::16]
+    charges_np, seeds = lines_64k_subset(128)
     length = 250
     n = seeds.shape[0]
-    charges = dev(w.charges)
-    eager_rows, eager_steps = ends_trace(gpu, w.charges, seeds, length)
+    charges = dev(charges_np)
+    eager_rows, eager_steps = ends_trace(gpu, charges_np, seeds, length)
+    w = gen.lines_8k()
     rows = gpu.seed_rows(dev(seeds), length)
     steps = torch.empty(n, dtype=torch.int32, device="cuda")
     g = torch.cuda.CUDAGraph()
