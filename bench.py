@@ -379,7 +379,9 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                 "d2h_s": perf_d["d2h_s"], "d2h_GB_per_s": perf_d["d2h_bytes"] / perf_d["d2h_s"] / 1e9,
                 "reference_gflops_formula": perf_d["gflops"],
                 "note": "em_calc_lines_f32 on fresh numpy arrays (best of 2 calls); the reference "
-                        "app's CalcField call, rows copied back into pageable memory"}}
+                        "app's CalcField call. Rows are traced in row chunks; each chunk's copy "
+                        "back into the pageable arrays (d2h_s sums the transfers) overlaps the "
+                        "next chunk's kernel"}}
         del rl, cl, sl
     # 7. particle step (Particle_System.cpp:108-190): 65,536 particles among the
     #    16,384 lines_64k charges, one step with the written-back state

@@ -195,9 +195,21 @@ struct HostPathBuffers {
     void* steps = nullptr;   // n step counts
     size_t steps_bytes = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy = nullptr;  // copies back of finished row chunks, beside the kernels
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t chunk_ev[8] = {};
 };
 static HostPathBuffers g_hp[64];
+
+// Row chunks of a host-array trace: the copy back of chunk k runs while the
+// kernel of chunk k+1 traces.  Possible for the plain and ref rules, whose
+// state is the point alone (a chunk continues from the previous chunk's last
+// row); one chunk per ~96 MB of output, at most 8.
+static uint32_t host_chunks(const StepP& sp, bool ends, size_t len, size_t out_bytes)
+{
+    if (ends || (sp.rule != kRulePlain && sp.rule != kRuleRef) || len < 16) return 1;
+    return (uint32_t)std::min<size_t>(8, std::max<size_t>(1, out_bytes / (96ull << 20)));
+}
 
 static int ensure(void** ptr, size_t* have, size_t need)
 {
@@ -249,7 +261,9 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     HostPathBuffers& b = g_hp[dev.device];
     if (!b.stream) {
         EM_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        EM_HIP(hipStreamCreateWithFlags(&b.copy, hipStreamNonBlocking));
         for (auto& e : b.ev) EM_HIP(hipEventCreate(&e));
+        for (auto& e : b.chunk_ev) EM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     const size_t cb = std::max<size_t>(p, 1) * sizeof(float4);
     const size_t row = n * sizeof(float);
@@ -275,30 +289,44 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     cfg.len = (uint32_t)len;
     const StepP sp = to_stepp(*st);
     uint32_t* dsteps = steps ? static_cast<uint32_t*>(b.steps) : nullptr;
-    if (ends)
+    const size_t out_bytes = (len - 1) * row;
+    const uint32_t chunks = host_chunks(sp, ends, len, 3 * out_bytes);
+    // chunk c covers rows bound(c) .. bound(c+1)-1
+    auto bound = [&](uint32_t c) { return (uint32_t)(1 + (uint64_t)(len - 1) * c / chunks); };
+    if (ends) {
         EM_TRY(launch_trace_ends(sh, dev, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
                                  dsteps, b.stream));
-    else
-        EM_TRY(launch_trace_direct(sh, cfg, static_cast<const float4*>(b.charges), dx, dy, dz, sp,
-                                   nullptr, 1, dsteps, b.stream));
+        EM_HIP(hipEventRecord(b.chunk_ev[0], b.stream));
+    } else {
+        for (uint32_t c = 0; c < chunks; ++c) {
+            Cfg part = cfg;
+            part.len = bound(c + 1);
+            EM_TRY(launch_trace_direct(sh, part, static_cast<const float4*>(b.charges), dx, dy, dz,
+                                       sp, nullptr, 1, dsteps, b.stream, bound(c)));
+            EM_HIP(hipEventRecord(b.chunk_ev[c], b.stream));
+        }
+    }
     EM_HIP(hipEventRecord(b.ev[2], b.stream));
-    const size_t out_bytes = (len - 1) * row;
-    {
-        // while the GPU traces: fault in the caller's output rows
-        const CopySeg out[3] = {{x + n, nullptr, out_bytes}, {y + n, nullptr, out_bytes},
-                                {z + n, nullptr, out_bytes}};
-        prefault_pages(out, 3);
+    double d2h = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const size_t r0 = bound(c), bytes = (size_t)(bound(c + 1) - r0) * row;
+        const size_t o = r0 * n;
+        const CopySeg segs[4] = {{x + o, dx + o, bytes}, {y + o, dy + o, bytes},
+                                 {z + o, dz + o, bytes}, {steps, dsteps, n * sizeof(uint32_t)}};
+        // while the GPU traces: fault in the caller's output rows of this chunk
+        prefault_pages(segs, 3);
+        EM_HIP(hipEventSynchronize(b.chunk_ev[c]));  // d2h_s: transfer time alone
+        EM_HIP(hipStreamWaitEvent(b.copy, b.chunk_ev[c], 0));
+        const double td = now_s();
+        EM_TRY(d2h_staged(dev.device, segs, (steps && c + 1 == chunks) ? 4 : 3, b.copy));
+        d2h += now_s() - td;
+        if (perf) perf->progress = (double)bound(c + 1) / (double)len;
     }
     EM_HIP(hipEventSynchronize(b.ev[2]));
-    const double td = now_s();
-    const CopySeg segs[4] = {{x + n, dx + n, out_bytes}, {y + n, dy + n, out_bytes},
-                             {z + n, dz + n, out_bytes}, {steps, dsteps, n * sizeof(uint32_t)}};
-    EM_TRY(d2h_staged(dev.device, segs, steps ? 4 : 3, b.stream));
-    const double d2h = now_s() - td;
 
     float ms[2] = {0, 0};
     EM_HIP(hipEventElapsedTime(&ms[0], b.ev[0], b.ev[1]));
-    EM_HIP(hipEventElapsedTime(&ms[1], b.ev[1], b.ev[2]));
+    EM_HIP(hipEventElapsedTime(&ms[1], b.ev[1], b.ev[2]));  // the kernels (chunks overlap copies)
     if (perf) {
         perf->time_s = now_s() - t0;
         perf->h2d_s = ms[0] * 1e-3;
@@ -363,7 +391,10 @@ int em_shutdown(void)
         if (b.steps) EM_HIP(hipFree(b.steps));
         for (auto& e : b.ev)
             if (e) EM_HIP(hipEventDestroy(e));
+        for (auto& e : b.chunk_ev)
+            if (e) EM_HIP(hipEventDestroy(e));
         if (b.stream) EM_HIP(hipStreamDestroy(b.stream));
+        if (b.copy) EM_HIP(hipStreamDestroy(b.copy));
         b = HostPathBuffers();
     }
     EM_HIP(hipSetDevice(cur));

@@ -80,9 +80,11 @@ int allow_lds(K kernel, size_t bytes)
 }
 
 // ---- kernels (defined in trace.hip / ends.hip / adjoint.hip) ---------------
+// Rows row0 .. cfg.len-1 (row0 > 1: plain and ref rules only, no
+// checkpoints; the trace continues from row row0-1).
 int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                         float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
-                        uint32_t* steps, hipStream_t stream);
+                        uint32_t* steps, hipStream_t stream, uint32_t row0 = 1);
 int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const Item* items,
                             uint32_t n_items, uint32_t* counter, const float4* charges, float* x,
                             float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,

@@ -29,9 +29,11 @@ __device__ __forceinline__ void store_ckpt(float* ckpt, uint64_t off, uint32_t m
     c[1] = make_float4(st.ux, st.uy, st.uz, st.alive ? 1.f : 0.f);
 }
 
-// Trace lines [line0, line0 + blockDim/S) of one configuration.
+// Trace lines [line0, line0 + blockDim/S) of one configuration, rows
+// row0 .. cfg.len-1 (row0 > 1 continues from row row0-1: the plain and ref
+// rules carry no state besides the point and the seed).
 template <int S>
-__device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
+__device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0, uint32_t row0,
                                             const float4* __restrict__ charges, float* __restrict__ X,
                                             float* __restrict__ Y, float* __restrict__ Z,
                                             const StepP& sp, float4* tile, uint32_t tile_cap,
@@ -52,18 +54,19 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0,
         __syncthreads();
     }
     const uint64_t base = cfg.line_off + li;
+    const uint64_t from = base + (uint64_t)(row0 - 1) * cfg.pitch;
     LineState st;
-    st.px = X[base];
-    st.py = Y[base];
-    st.pz = Z[base];
-    st.Px = st.px; st.Py = st.py; st.Pz = st.pz;
+    st.px = X[from];
+    st.py = Y[from];
+    st.pz = Z[from];
+    st.Px = X[base]; st.Py = Y[base]; st.Pz = Z[base];
     st.ux = st.uy = st.uz = 0.f;
     st.h = sp.h0;
     st.alive = valid;
-    st.steps = 0;
+    st.steps = row0 - 1;
     const bool writer = valid && k == 0;
     if (ckpt && writer) store_ckpt(ckpt, cfg.ckpt_off, 0, cfg.n, line, st);
-    for (uint32_t s = 1; s < cfg.len; ++s) {
+    for (uint32_t s = row0; s < cfg.len; ++s) {
         // Lines that ended keep writing their end point; the charge loop runs
         // while any line of the wave (of the block, when charges are re-staged
         // with barriers) is still moving.
@@ -98,11 +101,12 @@ __global__ void __launch_bounds__(512) k_trace_direct(Cfg cfg, const float4* __r
                                                       float* __restrict__ X, float* __restrict__ Y,
                                                       float* __restrict__ Z, StepP sp,
                                                       uint32_t tile_cap, float* __restrict__ ckpt,
-                                                      uint32_t ckpt_every, uint32_t* __restrict__ steps)
+                                                      uint32_t ckpt_every, uint32_t* __restrict__ steps,
+                                                      uint32_t row0)
 {
     extern __shared__ float4 tile[];
     const uint32_t line0 = blockIdx.x * (blockDim.x / S);
-    trace_block<S>(cfg, line0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
+    trace_block<S>(cfg, line0, row0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
 }
 
 template <int S>
@@ -125,7 +129,8 @@ __global__ void __launch_bounds__(512) k_trace_persistent(const Cfg* __restrict_
         if (it >= n_items) break;  // uniform across the block: every wave exits
         const Item item = items[it];
         const Cfg cfg = cfgs[item.cfg];
-        trace_block<S>(cfg, item.line0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
+        trace_block<S>(cfg, item.line0, 1u, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every,
+                       steps);
         __syncthreads();  // the next item re-stages the tile
     }
 }
@@ -176,14 +181,14 @@ template <int S>
 struct DirectL {
     static int run(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                    float* z, const StepP& sp, float* ckpt, uint32_t every, uint32_t* steps,
-                   hipStream_t stream)
+                   uint32_t row0, hipStream_t stream)
     {
         const uint32_t lpb = sh.block / S;
         const uint32_t grid = (cfg.n + lpb - 1) / lpb;
         const size_t lds = (size_t)sh.tile * sizeof(float4);
         EM_TRY(allow_lds(k_trace_direct<S>, lds));
         hipLaunchKernelGGL(k_trace_direct<S>, dim3(grid), dim3(sh.block), lds, stream, cfg, charges,
-                           x, y, z, sp, sh.tile, ckpt, every, steps);
+                           x, y, z, sp, sh.tile, ckpt, every, steps, row0);
         EM_HIP(hipGetLastError());
         return EM_OK;
     }
@@ -245,9 +250,13 @@ struct FieldL {
 
 int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                         float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
-                        uint32_t* steps, hipStream_t stream)
+                        uint32_t* steps, hipStream_t stream, uint32_t row0)
 {
-    return dispatch_S<DirectL>(sh.S, sh, cfg, charges, x, y, z, sp, ckpt, ckpt_every, steps, stream);
+    if (row0 < 1 || (row0 > 1 && (sp.rule == kRuleAngle || ckpt)))
+        EM_FAIL(EM_ERR_INVALID, "a trace may start past row 1 only for the plain and ref rules, "
+                                "without checkpoints");
+    return dispatch_S<DirectL>(sh.S, sh, cfg, charges, x, y, z, sp, ckpt, ckpt_every, steps, row0,
+                               stream);
 }
 
 int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const Item* items,

@@ -265,3 +265,19 @@ def test_no_charges_gives_the_reference_nan(gpu, orc):
     assert rc == 0
     assert np.array_equal(rows[:, 0], c_rows[:, 0])
     assert np.isnan(rows[:, 1:]).all() and np.isnan(c_rows[:, 1:]).all()
+
+
+@pytest.mark.parametrize("rule", [rules.ref(1.0), rules.plain(1.0)], ids=["ref", "plain"])
+def test_host_path_row_chunks_match_device_path(gpu, rule):
+    """The reference's default problem (1,024 charges, 16,384 lines, 2,500
+    rows) through the host-array entry point: its 491 MB of output is traced
+    in row chunks whose copies back overlap the next chunk's kernel.  Each
+    chunk continues from the previous chunk's last row, so the rows and step
+    counts are bit-identical to one device-side launch."""
+    w = gpu.legacy.preset("default")
+    rows_h, perf, steps_h = gpu.trace_host(w.charges, w.seeds, w.len, rule)
+    rows_d, steps_d = gpu.trace(dev(w.charges), dev(w.seeds), w.len, rule, with_steps=True)
+    torch.cuda.synchronize()
+    assert perf["progress"] == 1.0 and perf["kernel_s"] > 0
+    assert np.array_equal(steps_h.astype(np.int64), steps_d.cpu().numpy().astype(np.int64))
+    assert np.array_equal(rows_h, rows_d.cpu().numpy())
