@@ -190,19 +190,22 @@ __device__ __forceinline__ float group_sum(float v)
 // Full Coulomb sum at (px, py, pz) over p charges, using the LDS tile.
 // If p fits the tile (pp <= tile_cap) the tile is assumed already staged.
 // Every thread of the block must call this (it contains barriers when the
-// charges do not fit).
+// charges do not fit).  A wave none of whose lines needs the sum passes
+// sum = false (a wave-uniform value): it still stages and meets the barriers
+// but leaves the arithmetic, and its SIMD's issue slots, to the other waves;
+// its result is then zero.
 template <int S>
 __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t tile_cap,
                                          const float4* __restrict__ charges, uint32_t p,
                                          int k, float px, float py, float pz,
-                                         float& ex, float& ey, float& ez)
+                                         float& ex, float& ey, float& ez, bool sum = true)
 {
     FieldBody body;
     body.init(px, py, pz);
     constexpr uint32_t G = S * kGroup;
     const uint32_t pp = round_up(p, G);
     if (pp <= tile_cap) {
-        sum_tile<S>(tile, pp, k, body);
+        if (sum) sum_tile<S>(tile, pp, k, body);
     } else {
         for (uint32_t t0 = 0; t0 < p; t0 += tile_cap) {
             const uint32_t cnt = min(tile_cap, p - t0);
@@ -210,7 +213,7 @@ __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t til
             __syncthreads();
             stage(tile, charges + t0, cnt, cntp);
             __syncthreads();
-            sum_tile<S>(tile, cntp, k, body);
+            if (sum) sum_tile<S>(tile, cntp, k, body);
         }
     }
     ex = group_sum<S>(body.acc.sx + body.acc.cx);

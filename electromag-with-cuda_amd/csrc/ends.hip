@@ -141,7 +141,8 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
         const int any = single ? __any(active) : __syncthreads_or(active);
         if (__builtin_amdgcn_readfirstlane(any) == 0) break;  // uniform: a scalar branch
         float ex, ey, ez;
-        field_at<S>(tile, tile_cap, charges, p, k, st.px, st.py, st.pz, ex, ey, ez);
+        field_at<S>(tile, tile_cap, charges, p, k, st.px, st.py, st.pz, ex, ey, ez,
+                    __any(active) != 0);
         if (active) {
             step_update(sp, s, ex * sp.scale, ey * sp.scale, ez * sp.scale, st);
             if (st.alive) {

@@ -73,7 +73,8 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0, uint
         const bool need = single ? __any(st.alive) : __syncthreads_or(st.alive);
         if (need) {
             float ex, ey, ez;
-            field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez);
+            field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez,
+                        single || __any(st.alive) != 0);
             if (st.alive) {
                 ex *= sp.scale;
                 ey *= sp.scale;
