@@ -117,6 +117,14 @@ struct FieldTensorBody {
 
 // E (as field_at computes it), S3 and T at (px, py, pz).  Same staging and
 // barrier structure as field_at: every thread of the block calls it.
+// Records per pipelined group in the adjoint's two passes (the forward uses
+// kUnroll = 8): with 8 the lines pass holds 146 VGPRs and the charges pass
+// 154, three waves per SIMD; with 4, 114 and 105, four waves (fit_64
+// iteration 23.9 -> 23.1 ms); with 2 the lines pass stays at four waves and
+// loses ILP (adjoint_2k ratio 3.23 -> 3.39).  The sums consume every record
+// in order whatever the value, so results are bit for bit the same.
+constexpr int kTensorUnroll = 4;
+
 template <int S>
 __device__ __forceinline__ void field_tensor_at(float4* __restrict__ tile, uint32_t tile_cap,
                                                 const float4* __restrict__ charges, uint32_t p,
@@ -127,8 +135,10 @@ __device__ __forceinline__ void field_tensor_at(float4* __restrict__ tile, uint3
     body.init(px, py, pz);
     constexpr uint32_t G = S * kGroup;
     const uint32_t pp = round_up(p, G);
+    // four records per pipelined group: the tensor body's own registers
+    // would otherwise hold the lines pass to three waves per SIMD
     if (pp <= tile_cap) {
-        pipelined_tile<S, 1>(tile, pp, k, body);
+        pipelined_tile<S, 1, kTensorUnroll>(tile, pp, k, body);
     } else {
         for (uint32_t t0 = 0; t0 < p; t0 += tile_cap) {
             const uint32_t cnt = min(tile_cap, p - t0);
@@ -136,7 +146,7 @@ __device__ __forceinline__ void field_tensor_at(float4* __restrict__ tile, uint3
             __syncthreads();
             stage(tile, charges + t0, cnt, cntp);
             __syncthreads();
-            pipelined_tile<S, 1>(tile, cntp, k, body);
+            pipelined_tile<S, 1, kTensorUnroll>(tile, cntp, k, body);
         }
     }
     e[0] = group_sum<S>(body.acc.sx + body.acc.cx);
@@ -483,7 +493,7 @@ k_adjoint_charges(const Cfg* __restrict__ cfgs, const ChargeItem* __restrict__ i
             tile[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
-        pipelined_tile<1, 2>(tile, cntp, 0, body);
+        pipelined_tile<1, 2, kTensorUnroll>(tile, cntp, 0, body);  // two float4 per source
         if (base >= it.src_end) break;
     }
     if (valid) {

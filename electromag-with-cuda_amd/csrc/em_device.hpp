@@ -101,18 +101,21 @@ struct Acc {
 // of the next group are in flight (two register sets, no copies).
 // Body::term(const float4* rec) consumes one record into plain FP32 partial
 // sums; Body::fold() moves the partials into compensated totals every kBlk
-// records per lane and at the end of the tile.
-template <int S, int R, class Body>
+// records per lane and at the end of the tile.  U records per group (kUnroll
+// by default; a body that needs many registers of its own may take fewer):
+// every record is still consumed in order, so the sums do not depend on U.
+template <int S, int R, int U = kUnroll, class Body>
 __device__ __forceinline__ void pipelined_tile(const float4* __restrict__ tile, uint32_t count,
                                                int k, Body& body)
 {
-    const uint32_t ngroups = count / (S * kUnroll);  // even: count is a multiple of S*kGroup
+    static_assert(kGroup % (2 * U) == 0 && kBlk % U == 0, "U must divide the padding granule");
+    const uint32_t ngroups = count / (S * U);  // even: count is a multiple of S*kGroup
     const float4* t = tile + k * R;
-    constexpr uint32_t kStride = S * kUnroll * R;
-    constexpr uint32_t kFoldEvery = kBlk / kUnroll;  // groups per compensated block (even)
-    float4 a[kUnroll][R], b[kUnroll][R];
+    constexpr uint32_t kStride = S * U * R;
+    constexpr uint32_t kFoldEvery = kBlk / U;  // groups per compensated block (even)
+    float4 a[U][R], b[U][R];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int r = 0; r < R; ++r) a[u][r] = t[u * S * R + r];
     for (uint32_t g = 0; g < ngroups; g += 2) {
@@ -121,22 +124,22 @@ __device__ __forceinline__ void pipelined_tile(const float4* __restrict__ tile, 
         // next to their uses and expose the LDS latency every iteration).
         const float4* tb = t + (g + 1) * kStride;
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int r = 0; r < R; ++r) b[u][r] = tb[u * S * R + r];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) body.term(a[u]);
+        for (int u = 0; u < U; ++u) body.term(a[u]);
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t gn = min(g + 2, ngroups - 2);  // last pair re-reads itself
         const float4* ta = t + gn * kStride;
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int r = 0; r < R; ++r) a[u][r] = ta[u * S * R + r];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) body.term(b[u]);
+        for (int u = 0; u < U; ++u) body.term(b[u]);
         __builtin_amdgcn_sched_barrier(0);
         if ((g + 2) % kFoldEvery == 0 || g + 2 >= ngroups) body.fold();
     }
