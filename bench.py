@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--unroll", type=int, default=0, help="charge records per pipelined group (2, 4, 8)")
     ap.add_argument("--no-tune", action="store_true", help="use the library's heuristic shape")
     ap.add_argument("--cpu-lines", type=int, default=1024,
                     help="lines of the workload timed on the CPU (multiple of 64)")
@@ -142,8 +143,8 @@ def main():
     w = gen.lines_64k() if args.workload == "lines_64k" else gen.lines_8k()
     charges = torch.from_numpy(w.charges).cuda()
     seeds = torch.from_numpy(w.seeds).cuda()
-    if args.lanes or args.tile or args.block:
-        launch, tuned = emtrace.make_launch(args.lanes, args.tile, args.block), "given"
+    if args.lanes or args.tile or args.block or args.unroll:
+        launch, tuned = emtrace.make_launch(args.lanes, args.tile, args.block, args.unroll), "given"
     elif args.no_tune:
         launch, tuned = None, "heuristic"
     else:
@@ -161,7 +162,8 @@ def main():
     achieved = inter * FLOP_PER_INTERACTION / k_avg / 1e12
     finite = bool(torch.isfinite(rows).all().item())
     all_steps = bool((steps_buf == w.len - 1).all().item())
-    shape = (launch.lanes_per_line, launch.tile, launch.block) if launch is not None else None
+    shape = ((launch.lanes_per_line, launch.tile, launch.block, launch.unroll)
+             if launch is not None else None)
 
     out = {
         "metric": METRIC,
@@ -182,7 +184,7 @@ def main():
                                f"every line all {w.len - 1} steps",
                    "charges": w.p, "lines": w.n, "rows": w.len, "step_rule": "angle",
                    "launch": {"lanes_per_line": shape[0], "tile": shape[1], "block": shape[2],
-                              "source": tuned} if shape else {"source": tuned},
+                              "unroll": shape[3], "source": tuned} if shape else {"source": tuned},
                    "parallelism": "single GPU"},
         "line_steps_per_s": w.n * (w.len - 1) * args.steps / wall,
         "frac_fp32_peak": value * FLOP_PER_INTERACTION / (PEAK_FP32_TFLOPS * 1e12),
@@ -253,7 +255,7 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
             "useful_interactions_per_s": w.p * taken / kern[0], "kernel_ms": kern[0] * 1e3,
             "mean_steps": taken / w.n,
             "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
-                       "block": launch_s.block},
+                       "block": launch_s.block, "unroll": launch_s.unroll},
             "lines_past_first_phase": int((st > 192).sum().item()),
             "note": "lines end at sinks; two phases: lane groups take a new line whenever theirs "
                     "ends, up to 192 steps (launch = this phase's shape), then 256-lane "
@@ -274,7 +276,7 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
         res["batch_512"] = {
             "interactions": binter, "interactions_per_s": bval, "kernel_ms": min(kern) * 1e3,
             "ratio_to_plain": bval / plain_value, "configs": len(ws), "items": info["items"],
-            "grid": info["grid"], "launch": {k: info[k] for k in ("lanes_per_line", "tile", "block")}}
+            "grid": info["grid"], "launch": {k: info[k] for k in ("lanes_per_line", "tile", "block", "unroll")}}
         plan.close()
         del blines, bch
     # 3. differentiation workload: forward vs forward + adjoint

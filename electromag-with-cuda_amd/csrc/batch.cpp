@@ -262,6 +262,7 @@ int em_batch_info(const em_batch* plan, double* interactions, int32_t* items, in
     if (used) {
         used->lanes_per_line = plan->sh.S;
         used->tile = (int32_t)plan->sh.tile;
+        used->unroll = plan->sh.unroll;
         used->block = plan->sh.block;
     }
     return EM_OK;

@@ -68,6 +68,8 @@ static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (
 int validate_shape(const DeviceInfo& dev, const Shape& s)
 {
     if (!pow2_in(s.S, 1, 32)) EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1..32}", s.S);
+    if (s.unroll != 2 && s.unroll != 4 && s.unroll != 8)
+        EM_FAIL(EM_ERR_INVALID, "unroll %d not in {2, 4, 8}", s.unroll);
     if (!pow2_in(s.block, 64, 512)) EM_FAIL(EM_ERR_INVALID, "block %d not in {64..512}", s.block);
     if (s.tile == 0 || s.tile % (uint32_t)(s.S * kGroup) || s.tile % 256)
         EM_FAIL(EM_ERR_INVALID, "tile %u not a positive multiple of 256 and of %d", s.tile, s.S * kGroup);
@@ -90,6 +92,18 @@ static uint32_t tile_for(const DeviceInfo& dev, uint64_t p, int S, int resident)
     return (uint32_t)std::min(t, tmax);
 }
 
+// Charge records per pipelined group (the trace kernel takes 100 VGPRs at 8,
+// 68 at 4, 52 at 2; the rows do not depend on it).  Measured on MI355X: with
+// 512-thread blocks and no line ends, 2 is fastest (16,384 charges x 65,536
+// lines, 4 lanes per line: 831 ms against 917 ms at 8; the 512-configuration
+// batch: 451 against 464 ms); with 256-thread blocks or line ends, 8 is
+// fastest or within 2 % (the same batch at 256 threads: 467 against 519 ms at
+// 2; the 64-configuration checkpointed forward: 13.1 against 14.5 ms).
+static int default_unroll(const Shape& s, bool stop)
+{
+    return (s.block >= 512 && !stop) ? 2 : 8;
+}
+
 int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
                  bool stop, bool refill)
 {
@@ -106,6 +120,7 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
         while (block > 64 && n * (uint64_t)s.S / (uint64_t)block < (uint64_t)dev.cus) block /= 2;
         s.block = (req && req->block) ? req->block : block;
         s.tile = (req && req->tile) ? (uint32_t)req->tile : tile_for(dev, p, s.S, 2);
+        s.unroll = (req && req->unroll) ? req->unroll : default_unroll(s, true);
         EM_TRY(validate_shape(dev, s));
         out = s;
         return EM_OK;
@@ -137,6 +152,7 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
                                                       std::max(1, 1024 / s.block));
         s.tile = tile_for(dev, p, s.S, resident);
     }
+    s.unroll = (req && req->unroll) ? req->unroll : default_unroll(s, stop);
     EM_TRY(validate_shape(dev, s));
     out = s;
     return EM_OK;
@@ -339,6 +355,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
         perf->lanes_per_line = sh.S;
         perf->tile = (int32_t)sh.tile;
         perf->block = sh.block;
+        perf->unroll = sh.unroll;
         perf->progress = 1.0;
     }
     return EM_OK;

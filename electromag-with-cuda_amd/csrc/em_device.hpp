@@ -166,11 +166,11 @@ struct FieldBody {
     }
 };
 
-template <int S>
+template <int S, int U = kUnroll>
 __device__ __forceinline__ void sum_tile(const float4* __restrict__ tile, uint32_t count, int k,
                                          FieldBody& body)
 {
-    pipelined_tile<S, 1>(tile, count, k, body);
+    pipelined_tile<S, 1, U>(tile, count, k, body);
 }
 
 // Cooperative global -> LDS copy of `cnt` charge records, padded to `cntp`.
@@ -197,7 +197,7 @@ __device__ __forceinline__ float group_sum(float v)
 // sum = false (a wave-uniform value): it still stages and meets the barriers
 // but leaves the arithmetic, and its SIMD's issue slots, to the other waves;
 // its result is then zero.
-template <int S>
+template <int S, int U = kUnroll>
 __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t tile_cap,
                                          const float4* __restrict__ charges, uint32_t p,
                                          int k, float px, float py, float pz,
@@ -208,7 +208,7 @@ __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t til
     constexpr uint32_t G = S * kGroup;
     const uint32_t pp = round_up(p, G);
     if (pp <= tile_cap) {
-        if (sum) sum_tile<S>(tile, pp, k, body);
+        if (sum) sum_tile<S, U>(tile, pp, k, body);
     } else {
         for (uint32_t t0 = 0; t0 < p; t0 += tile_cap) {
             const uint32_t cnt = min(tile_cap, p - t0);
@@ -216,7 +216,7 @@ __device__ __forceinline__ void field_at(float4* __restrict__ tile, uint32_t til
             __syncthreads();
             stage(tile, charges + t0, cnt, cntp);
             __syncthreads();
-            if (sum) sum_tile<S>(tile, cntp, k, body);
+            if (sum) sum_tile<S, U>(tile, cntp, k, body);
         }
     }
     ex = group_sum<S>(body.acc.sx + body.acc.cx);

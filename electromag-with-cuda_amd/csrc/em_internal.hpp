@@ -53,6 +53,7 @@ struct Shape {
     int S = 1;         // lanes per line
     int block = 256;   // threads per workgroup
     uint32_t tile = 0; // charges per LDS tile
+    int unroll = kUnroll;  // charge records per pipelined group: 2, 4 or 8
 };
 
 // Heuristic launch shape for one configuration (p charges, n lines); fields

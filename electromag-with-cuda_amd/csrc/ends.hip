@@ -7,7 +7,7 @@
 // launch shape cannot serve both: groups of S lanes per line keep a wave busy
 // until its slowest line ends, and a line that never ends is then traced at
 // the rate of S lanes.  Here:
-//   1. k_trace_ends<S>: a persistent grid of S-lane groups.  A group whose
+//   1. k_trace_ends<S, U>: a persistent grid of S-lane groups.  A group whose
 //      line ends takes the next line from a counter; a line still moving
 //      after kEndsCap steps is set aside (its state goes to a list) and the
 //      group moves on.  Charges go through LDS tiles as in the other kernels.
@@ -72,7 +72,7 @@ __global__ void k_ends_prepare(EndsCtrl* __restrict__ ctrl, const float4* __rest
 // phase holds at once (more than `tail_max`), this kernel takes them instead,
 // from step `start` + 1 to the end, with the same refill; otherwise it exits
 // at once (the second phase has them).
-template <int S, bool RESUME>
+template <int S, int U, bool RESUME>
 __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __restrict__ charges,
                                                     float* __restrict__ X, float* __restrict__ Y,
                                                     float* __restrict__ Z, StepP sp,
@@ -141,8 +141,8 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
         const int any = single ? __any(active) : __syncthreads_or(active);
         if (__builtin_amdgcn_readfirstlane(any) == 0) break;  // uniform: a scalar branch
         float ex, ey, ez;
-        field_at<S>(tile, tile_cap, charges, p, k, st.px, st.py, st.pz, ex, ey, ez,
-                    __any(active) != 0);
+        field_at<S, U>(tile, tile_cap, charges, p, k, st.px, st.py, st.pz, ex, ey, ez,
+                       __any(active) != 0);
         if (active) {
             step_update(sp, s, ex * sp.scale, ey * sp.scale, ez * sp.scale, st);
             if (st.alive) {
@@ -438,23 +438,23 @@ __global__ void k_ends_fill(Cfg cfg, float* __restrict__ X, float* __restrict__ 
     }
 }
 
-template <int S, bool RESUME>
+template <int S, int U, bool RESUME>
 int launch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
                   float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t start,
                   uint32_t tail_max, uint32_t* steps, EndsCtrl* ctrl, float4* surv,
                   hipStream_t stream)
 {
     const size_t lds = (size_t)sh.tile * sizeof(float4);
-    EM_TRY(allow_lds(k_trace_ends<S, RESUME>, lds));
+    EM_TRY(allow_lds(k_trace_ends<S, U, RESUME>, lds));
     int per_cu = 0;
-    EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_ends<S, RESUME>, sh.block,
-                                                         lds));
+    EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_ends<S, U, RESUME>,
+                                                         sh.block, lds));
     if (per_cu < 1)
         EM_FAIL(EM_ERR_INVALID, "launch shape does not fit a CU (block %d, tile %u)", sh.block, sh.tile);
     const uint64_t groups_per_block = (uint64_t)sh.block / S;
     const uint64_t want = (cfg.n + groups_per_block - 1) / groups_per_block;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * dev.cus, want);
-    hipLaunchKernelGGL((k_trace_ends<S, RESUME>), dim3(grid), dim3(sh.block), lds, stream, cfg,
+    hipLaunchKernelGGL((k_trace_ends<S, U, RESUME>), dim3(grid), dim3(sh.block), lds, stream, cfg,
                        charges, x, y, z, sp, sh.tile, cap, start, tail_max, steps, ctrl, surv);
     EM_HIP(hipGetLastError());
     return EM_OK;
@@ -466,10 +466,18 @@ int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, cons
                     uint32_t tail_max, uint32_t* steps, EndsCtrl* ctrl, float4* surv,
                     hipStream_t stream)
 {
-#define EM_PHASE1(SS)                                                                            \
-    case SS:                                                                                     \
-        return launch_phase1<SS, RESUME>(sh, dev, cfg, charges, x, y, z, sp, cap, start, tail_max, \
-                                         steps, ctrl, surv, stream)
+#define EM_PHASE1_U(SS, UU)                                                                   \
+    case UU:                                                                                  \
+        return launch_phase1<SS, UU, RESUME>(sh, dev, cfg, charges, x, y, z, sp, cap, start,   \
+                                             tail_max, steps, ctrl, surv, stream)
+#define EM_PHASE1(SS)                                                                         \
+    case SS:                                                                                  \
+        switch (sh.unroll) {                                                                  \
+            EM_PHASE1_U(SS, 2);                                                               \
+            EM_PHASE1_U(SS, 4);                                                               \
+            EM_PHASE1_U(SS, 8);                                                               \
+            default: EM_FAIL(EM_ERR_INVALID, "unroll %d not in {2, 4, 8}", sh.unroll);        \
+        }
     switch (sh.S) {
         EM_PHASE1(1);
         EM_PHASE1(2);
@@ -480,6 +488,7 @@ int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, cons
         default: EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1,2,4,8,16,32}", sh.S);
     }
 #undef EM_PHASE1
+#undef EM_PHASE1_U
 }
 
 // Per-device work memory of the two-phase trace: counters, step counts (when

@@ -5,9 +5,9 @@
 // charges) and in its dormant OpenCL design (GPGPU_Segment/src/
 // Electrostatics.cl.c:282-367, one work-item per line, 64-128-charge __local
 // tiles) is done here as:
-//   * k_trace_direct<S>     -- one workgroup per block of lines of a single
+//   * k_trace_direct<S, U>  -- one workgroup per block of lines of a single
 //                              configuration; all len-1 steps in one launch;
-//   * k_trace_persistent<S> -- a persistent grid that pulls (configuration,
+//   * k_trace_persistent<S, U> -- a persistent grid that pulls (configuration,
 //                              line-block) items, largest first, from an atomic
 //                              counter (the batched form);
 //   * k_field<S>            -- the Coulomb sum alone at arbitrary points.
@@ -32,7 +32,7 @@ __device__ __forceinline__ void store_ckpt(float* ckpt, uint64_t off, uint32_t m
 // Trace lines [line0, line0 + blockDim/S) of one configuration, rows
 // row0 .. cfg.len-1 (row0 > 1 continues from row row0-1: the plain and ref
 // rules carry no state besides the point and the seed).
-template <int S>
+template <int S, int U>
 __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0, uint32_t row0,
                                             const float4* __restrict__ charges, float* __restrict__ X,
                                             float* __restrict__ Y, float* __restrict__ Z,
@@ -73,8 +73,8 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0, uint
         const bool need = single ? __any(st.alive) : __syncthreads_or(st.alive);
         if (need) {
             float ex, ey, ez;
-            field_at<S>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez,
-                        single || __any(st.alive) != 0);
+            field_at<S, U>(tile, tile_cap, ch, p, k, st.px, st.py, st.pz, ex, ey, ez,
+                           single || __any(st.alive) != 0);
             if (st.alive) {
                 ex *= sp.scale;
                 ey *= sp.scale;
@@ -97,7 +97,7 @@ __device__ __forceinline__ void trace_block(const Cfg& cfg, uint32_t line0, uint
     if (writer && steps_out) steps_out[cfg.line_index + line] = st.steps;
 }
 
-template <int S>
+template <int S, int U>
 __global__ void __launch_bounds__(512) k_trace_direct(Cfg cfg, const float4* __restrict__ charges,
                                                       float* __restrict__ X, float* __restrict__ Y,
                                                       float* __restrict__ Z, StepP sp,
@@ -107,10 +107,10 @@ __global__ void __launch_bounds__(512) k_trace_direct(Cfg cfg, const float4* __r
 {
     extern __shared__ float4 tile[];
     const uint32_t line0 = blockIdx.x * (blockDim.x / S);
-    trace_block<S>(cfg, line0, row0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
+    trace_block<S, U>(cfg, line0, row0, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every, steps);
 }
 
-template <int S>
+template <int S, int U>
 __global__ void __launch_bounds__(512) k_trace_persistent(const Cfg* __restrict__ cfgs,
                                                           const Item* __restrict__ items,
                                                           uint32_t n_items, uint32_t* counter,
@@ -130,8 +130,8 @@ __global__ void __launch_bounds__(512) k_trace_persistent(const Cfg* __restrict_
         if (it >= n_items) break;  // uniform across the block: every wave exits
         const Item item = items[it];
         const Cfg cfg = cfgs[item.cfg];
-        trace_block<S>(cfg, item.line0, 1u, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every,
-                       steps);
+        trace_block<S, U>(cfg, item.line0, 1u, charges, X, Y, Z, sp, tile, tile_cap, ckpt, ckpt_every,
+                          steps);
         __syncthreads();  // the next item re-stages the tile
     }
 }
@@ -164,6 +164,34 @@ __global__ void __launch_bounds__(512) k_field(const float4* __restrict__ charge
     }
 }
 
+// Lanes per line S and records per pipelined group U (2, 4 or 8: the sums
+// consume the records in the same order whatever U, so U changes the speed
+// only, not the bits).
+template <template <int, int> class F, int S, typename... A>
+int dispatch_U(int U, A&&... a)
+{
+    switch (U) {
+        case 2: return F<S, 2>::run(a...);
+        case 4: return F<S, 4>::run(a...);
+        case 8: return F<S, 8>::run(a...);
+        default: EM_FAIL(EM_ERR_INVALID, "unroll %d not in {2, 4, 8}", U);
+    }
+}
+
+template <template <int, int> class F, typename... A>
+int dispatch_SU(int S, int U, A&&... a)
+{
+    switch (S) {
+        case 1: return dispatch_U<F, 1>(U, a...);
+        case 2: return dispatch_U<F, 2>(U, a...);
+        case 4: return dispatch_U<F, 4>(U, a...);
+        case 8: return dispatch_U<F, 8>(U, a...);
+        case 16: return dispatch_U<F, 16>(U, a...);
+        case 32: return dispatch_U<F, 32>(U, a...);
+        default: EM_FAIL(EM_ERR_INVALID, "lanes_per_line %d not in {1,2,4,8,16,32}", S);
+    }
+}
+
 template <template <int> class F, typename... A>
 int dispatch_S(int S, A&&... a)
 {
@@ -178,7 +206,7 @@ int dispatch_S(int S, A&&... a)
     }
 }
 
-template <int S>
+template <int S, int U>
 struct DirectL {
     static int run(const Shape& sh, const Cfg& cfg, const float4* charges, float* x, float* y,
                    float* z, const StepP& sp, float* ckpt, uint32_t every, uint32_t* steps,
@@ -187,15 +215,15 @@ struct DirectL {
         const uint32_t lpb = sh.block / S;
         const uint32_t grid = (cfg.n + lpb - 1) / lpb;
         const size_t lds = (size_t)sh.tile * sizeof(float4);
-        EM_TRY(allow_lds(k_trace_direct<S>, lds));
-        hipLaunchKernelGGL(k_trace_direct<S>, dim3(grid), dim3(sh.block), lds, stream, cfg, charges,
-                           x, y, z, sp, sh.tile, ckpt, every, steps, row0);
+        EM_TRY(allow_lds(k_trace_direct<S, U>, lds));
+        hipLaunchKernelGGL((k_trace_direct<S, U>), dim3(grid), dim3(sh.block), lds, stream, cfg,
+                           charges, x, y, z, sp, sh.tile, ckpt, every, steps, row0);
         EM_HIP(hipGetLastError());
         return EM_OK;
     }
 };
 
-template <int S>
+template <int S, int U>
 struct PersistentL {
     static int run(const Shape& sh, int grid, const Cfg* cfgs, const Item* items, uint32_t n_items,
                    uint32_t* counter, const float4* charges, float* x, float* y, float* z,
@@ -203,25 +231,25 @@ struct PersistentL {
                    hipStream_t stream)
     {
         const size_t lds = (size_t)sh.tile * sizeof(float4);
-        EM_TRY(allow_lds(k_trace_persistent<S>, lds));
+        EM_TRY(allow_lds(k_trace_persistent<S, U>, lds));
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
         EM_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_trace_persistent<S>, dim3(grid), dim3(sh.block), lds, stream, cfgs,
-                           items, n_items, counter, charges, x, y, z, sp, sh.tile, ckpt, every,
-                           steps);
+        hipLaunchKernelGGL((k_trace_persistent<S, U>), dim3(grid), dim3(sh.block), lds, stream,
+                           cfgs, items, n_items, counter, charges, x, y, z, sp, sh.tile, ckpt,
+                           every, steps);
         EM_HIP(hipGetLastError());
         return EM_OK;
     }
 };
 
-template <int S>
+template <int S, int U>
 struct GridL {
     static int run(const Shape& sh, const DeviceInfo& dev, int* grid)
     {
         int per_cu = 0;
         const size_t lds = (size_t)sh.tile * sizeof(float4);
-        EM_TRY(allow_lds(k_trace_persistent<S>, lds));
-        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<S>,
+        EM_TRY(allow_lds(k_trace_persistent<S, U>, lds));
+        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<S, U>,
                                                              sh.block, lds));
         if (per_cu < 1) EM_FAIL(EM_ERR_INVALID, "launch shape does not fit a CU (block %d, tile %u)",
                                 sh.block, sh.tile);
@@ -256,8 +284,8 @@ int launch_trace_direct(const Shape& sh, const Cfg& cfg, const float4* charges, 
     if (row0 < 1 || (row0 > 1 && (sp.rule == kRuleAngle || ckpt)))
         EM_FAIL(EM_ERR_INVALID, "a trace may start past row 1 only for the plain and ref rules, "
                                 "without checkpoints");
-    return dispatch_S<DirectL>(sh.S, sh, cfg, charges, x, y, z, sp, ckpt, ckpt_every, steps, row0,
-                               stream);
+    return dispatch_SU<DirectL>(sh.S, sh.unroll, sh, cfg, charges, x, y, z, sp, ckpt, ckpt_every,
+                                steps, row0, stream);
 }
 
 int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const Item* items,
@@ -265,13 +293,13 @@ int launch_trace_persistent(const Shape& sh, int grid, const Cfg* cfgs, const It
                             float* y, float* z, const StepP& sp, float* ckpt, uint32_t ckpt_every,
                             uint32_t* steps, hipStream_t stream)
 {
-    return dispatch_S<PersistentL>(sh.S, sh, grid, cfgs, items, n_items, counter, charges, x, y, z,
-                                   sp, ckpt, ckpt_every, steps, stream);
+    return dispatch_SU<PersistentL>(sh.S, sh.unroll, sh, grid, cfgs, items, n_items, counter,
+                                    charges, x, y, z, sp, ckpt, ckpt_every, steps, stream);
 }
 
 int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid)
 {
-    return dispatch_S<GridL>(sh.S, sh, dev, grid);
+    return dispatch_SU<GridL>(sh.S, sh.unroll, sh, dev, grid);
 }
 
 int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,

@@ -337,6 +337,7 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
         perf->lanes_per_line = 1;
         perf->tile = (int32_t)(p <= kTile64 ? std::max<size_t>(p, 1) : kTile64 / 2);
         perf->block = kBlock64;
+        perf->unroll = 0;  // the FP64 kernel's loop has no pipelined groups
         perf->progress = 1.0;
     }
     return EM_OK;

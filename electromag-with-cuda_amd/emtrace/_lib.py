@@ -43,7 +43,7 @@ class em_step(ctypes.Structure):
 
 class em_launch(ctypes.Structure):
     _fields_ = [("lanes_per_line", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("block", ctypes.c_int32)]
+                ("block", ctypes.c_int32), ("unroll", ctypes.c_int32)]
 
 
 class em_perf(ctypes.Structure):
@@ -53,7 +53,7 @@ class em_perf(ctypes.Structure):
                 ("interactions", ctypes.c_double), ("interactions_per_s", ctypes.c_double),
                 ("line_steps_per_s", ctypes.c_double), ("gflops", ctypes.c_double),
                 ("progress", ctypes.c_double), ("lanes_per_line", ctypes.c_int32),
-                ("tile", ctypes.c_int32), ("block", ctypes.c_int32)]
+                ("tile", ctypes.c_int32), ("block", ctypes.c_int32), ("unroll", ctypes.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -26,8 +26,11 @@ def make_step(rule: StepRule) -> em_step:
                    float(rule.hmax), float(rule.turn), int(bool(rule.stop)))
 
 
-def make_launch(lanes_per_line: int = 0, tile: int = 0, block: int = 0) -> em_launch:
-    return em_launch(int(lanes_per_line), int(tile), int(block))
+def make_launch(lanes_per_line: int = 0, tile: int = 0, block: int = 0,
+                unroll: int = 0) -> em_launch:
+    """A launch shape; 0 leaves a field to the library.  ``unroll`` (2, 4 or 8
+    charge records per pipelined group) changes the speed only, never the bits."""
+    return em_launch(int(lanes_per_line), int(tile), int(block), int(unroll))
 
 
 def _launch_arg(launch):
@@ -321,7 +324,8 @@ class BatchPlan:
         call("em_batch_info", self._h, ctypes.byref(inter), ctypes.byref(items), ctypes.byref(grid),
              ctypes.byref(used))
         return {"interactions": inter.value, "items": items.value, "grid": grid.value,
-                "lanes_per_line": used.lanes_per_line, "tile": used.tile, "block": used.block}
+                "lanes_per_line": used.lanes_per_line, "tile": used.tile, "block": used.block,
+                "unroll": used.unroll}
 
     def run(self, charges, lines, steps=None):
         """Trace every configuration; ``steps`` (optional int32 device tensor of

@@ -1,9 +1,11 @@
 """Launch-shape autotuner for the tracer.
 
 A launch shape is (lanes per line S, threads per workgroup, charges per LDS
-tile).  The tuner times each candidate on the caller's own inputs with a
-reduced row count (the per-step cost is what differs between shapes) and
-keeps the fastest.  Results are cached in a JSON file keyed by device,
+tile, unroll U = charge records per pipelined group).  The tuner times each
+(S, block, tile) candidate on the caller's own inputs with a reduced row count
+(the per-step cost is what differs between shapes) at the library's default
+unroll, then the fastest of them at U = 8, 4 and 2, and keeps the fastest.
+Every shape gives bit-identical rows; only the time differs.  Results are cached in a JSON file keyed by device,
 problem size bucket and step rule (``$EMTRACE_TUNE_CACHE``, default
 ``~/.cache/emtrace/tune.json``); a small table of shapes measured on MI355X
 (``SHIPPED``) answers when the cache has no entry.
@@ -64,28 +66,68 @@ def _device_key() -> str:
 
 # Shapes measured by full sweeps on MI355X (gfx950, 256 CUs), used when the
 # local cache has no entry: the fastest shape of each sweep, ties (within
-# 1%) going to fewer lanes per line.  [S, block, tile, ms at 48 rows]
+# 1%) going to fewer lanes per line.  [S, block, tile, U, ms at 48 rows]
 SHIPPED = {
-    # 16,384 charges, 65,536 lines, angle rule: all steps; with line ends the
-    # first phase of the two-phase trace (51 ms in all; with 16 lanes per line
-    # 57 ms, with 8 lanes 76 ms, with one-wave blocks over 512-charge tiles 70 ms)
-    "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, None],
-    "plain:gfx950:256:16384:65536:2:1": [32, 512, 4096, None],
+    # 16,384 charges, 65,536 lines, angle rule: all steps (2,500 rows: 831 ms;
+    # unroll 4: 847 ms, 8: 917 ms; 256-thread blocks: 984 ms at best); with
+    # line ends the first phase of the two-phase trace (49-51 ms in all, any
+    # unroll within 2 %; with 16 lanes per line 53-57 ms, with 8 lanes 76 ms,
+    # with one-wave blocks over 512-charge tiles 70 ms)
+    "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, 2, None],
+    "plain:gfx950:256:16384:65536:2:1": [32, 512, 4096, 8, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule
-    "batch:gfx950:256:512:8192:1048576:2:0": [4, 256, 4096, None],
+    # (1,000 rows: 451 ms; 256-thread blocks at unroll 8: 467 ms)
+    "batch:gfx950:256:512:8192:1048576:2:0": [4, 512, 4096, 2, None],
 }
 
 TIE = 1.01  # timings within 1% of the best count as ties
+UNROLLS = (8, 4, 2)  # second stage, in tie-break order
 
 
 def _pick(report):
-    """Fastest candidate; among ties, the earliest (fewest lanes per line)."""
-    best = min(r[3] for r in report)
-    return next(r for r in report if r[3] <= best * TIE)
+    """Fastest candidate of rows (S, block, tile, U, ms, ...); among ties, the
+    earliest (fewest lanes per line; in the unroll stage, the larger U)."""
+    best = min(r[4] for r in report)
+    return next(r for r in report if r[4] <= best * TIE)
 
 
 def _lookup(key: str):
-    return _load().get(key) or SHIPPED.get(key)
+    """Cached or shipped [S, block, tile, U, ms]; an entry written before the
+    unroll was tuned ([S, block, tile, ms]) reads as U = 0 (library default)."""
+    hit = _load().get(key) or SHIPPED.get(key)
+    if hit and len(hit) == 4:
+        hit = hit[:3] + [0] + hit[3:]
+    return hit
+
+
+def _launch(row):
+    S, block, tile, U = row[:4]
+    return api.make_launch(S, tile, block, U)
+
+
+def _timed_row(S, block, tile, U, run):
+    """(S, block, tile, U, ms), or ms = inf and the error for a shape the
+    device cannot hold."""
+    try:
+        return (S, block, tile, U, run(api.make_launch(S, tile, block, U)))
+    except Exception as e:
+        return (S, block, tile, U, float("inf"), str(e)[:80])
+
+
+def _tune(shapes, run):
+    """Stage 1: every (S, block, tile) at the default unroll; stage 2: the
+    fastest shape at each of UNROLLS.  Returns (best row, report)."""
+    report = [_timed_row(S, block, tile, 0, run) for S, block, tile in shapes]
+    first = _pick(report)
+    if first[4] == float("inf"):
+        return first, report
+    second = [_timed_row(first[0], first[1], first[2], U, run) for U in UNROLLS]
+    report += second
+    ok = [r for r in second if r[4] < float("inf")]
+    best = _pick(ok) if ok else first
+    if best[4] * TIE > first[4]:
+        best = first  # the default unroll stays unless an explicit one beats it by the margin
+    return best, report
 
 
 def _bucket(v: int) -> int:
@@ -161,7 +203,7 @@ def tune_plain(charges, seeds, rule: StepRule, rows: Optional[int] = None, use_c
 
     Candidates are timed on ``rows`` rows (48; 400 with line ends, so that the
     lines' different lengths show).  Returns (em_launch, report) where report
-    lists (S, block, tile, ms)."""
+    lists (S, block, tile, U, ms); U = 0 is the library's default unroll."""
     import torch
     p, n = int(charges.shape[0]), int(seeds.shape[0])
     if rows is None:
@@ -170,24 +212,16 @@ def tune_plain(charges, seeds, rule: StepRule, rows: Optional[int] = None, use_c
     if use_cache:
         hit = _lookup(key)
         if hit:
-            return api.make_launch(hit[0], hit[2], hit[1]), [tuple(hit) + ("cached",)]
+            return _launch(hit), [tuple(hit) + ("cached",)]
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
     cands = candidates(p, n, cus, stop=rule.stop)
     if max_candidates:
         cands = cands[:max_candidates]
     buf = api.seed_rows(seeds, rows)
-    report = []
-    for S, block, tile in cands:
-        launch = api.make_launch(S, tile, block)
-        try:
-            ms = _time(lambda: api.trace_into(charges, buf, n, rule, launch))
-        except Exception as e:  # a shape the device cannot hold
-            report.append((S, block, tile, float("inf"), str(e)[:80]))
-            continue
-        report.append((S, block, tile, ms))
-    best = _pick(report)
-    _store(key, list(best[:4]))
-    return api.make_launch(best[0], best[2], best[1]), report
+    best, report = _tune(cands, lambda launch: _time(
+        lambda: api.trace_into(charges, buf, n, rule, launch)))
+    _store(key, list(best[:5]))
+    return _launch(best), report
 
 
 def tune_batch(workloads, rule: StepRule, rows: int = 48, use_cache: bool = True):
@@ -201,26 +235,24 @@ def tune_batch(workloads, rule: StepRule, rows: int = 48, use_cache: bool = True
     if use_cache:
         hit = _lookup(key)
         if hit:
-            return api.make_launch(hit[0], hit[2], hit[1]), [tuple(hit) + ("cached",)]
+            return _launch(hit), [tuple(hit) + ("cached",)]
     cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
     short = [type(w)(w.charges, w.seeds, rows) for w in workloads]
     layout, charges, lines = api.pack_batch(short)
-    report = []
-    seen = set()
+    shapes = []
     for S, block, tile in candidates(max(ps), sum(ns), cus, batch=True):
         # persistent grid: several workgroups per CU keep every SIMD busy
         for t in sorted({tile, min(tile, _round_up(max(ps), max(256, S * GROUP)))}):
-            if (S, block, t) in seen:
-                continue
-            seen.add((S, block, t))
-            try:
-                plan = api.BatchPlan(layout, rule, api.make_launch(S, t, block), use_graph=False)
-                ms = _time(lambda: plan.run(charges, lines))
-                plan.close()
-            except Exception as e:
-                report.append((S, block, t, float("inf"), str(e)[:80]))
-                continue
-            report.append((S, block, t, ms))
-    best = _pick(report)
-    _store(key, list(best[:4]))
-    return api.make_launch(best[0], best[2], best[1]), report
+            if (S, block, t) not in shapes:
+                shapes.append((S, block, t))
+
+    def run(launch):
+        plan = api.BatchPlan(layout, rule, launch, use_graph=False)
+        try:
+            return _time(lambda: plan.run(charges, lines))
+        finally:
+            plan.close()
+
+    best, report = _tune(shapes, run)
+    _store(key, list(best[:5]))
+    return _launch(best), report

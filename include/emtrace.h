@@ -61,6 +61,8 @@ typedef struct em_launch {
     int32_t lanes_per_line; /* 1, 2, 4, 8, 16 or 32 lanes split one line's charge sum */
     int32_t tile;           /* charges per LDS tile (multiple of 64)                  */
     int32_t block;          /* threads per workgroup: 64, 128, 256 or 512             */
+    int32_t unroll;         /* charge records per lane per pipelined group: 2, 4 or 8;
+                               any value gives the same bits (speed only)             */
 } em_launch;
 
 /* Timing/performance record -- the fields of the reference's perfPacket
@@ -77,7 +79,7 @@ typedef struct em_perf {
     double gflops;             /* n*(len-1)*(p*18+13)/time_s/1e9, the reference
                                   formula (CPU_Implement.cpp:347-349)          */
     volatile double progress;  /* 0 at entry, 1.0 when the rows are on the host */
-    int32_t lanes_per_line, tile, block; /* launch shape actually used     */
+    int32_t lanes_per_line, tile, block, unroll; /* launch shape actually used */
 } em_perf;
 
 /* ---- library lifetime ----------------------------------------------------*/

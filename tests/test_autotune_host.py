@@ -19,14 +19,44 @@ def test_candidates_fit_lds_and_granules():
 
 
 def test_pick_prefers_fewer_lanes_among_ties():
-    rep = [(1, 256, 512, 10.5), (2, 256, 512, 10.04), (4, 512, 512, 10.0), (8, 512, 512, 9.99)]
+    rep = [(1, 256, 512, 0, 10.5), (2, 256, 512, 0, 10.04), (4, 512, 512, 0, 10.0),
+           (8, 512, 512, 0, 9.99)]
     assert autotune._pick(rep)[:3] == (2, 256, 512)
-    rep = [(1, 256, 512, 12.0), (4, 512, 512, 10.0), (8, 512, 512, float("inf"))]
+    rep = [(1, 256, 512, 0, 12.0), (4, 512, 512, 0, 10.0), (8, 512, 512, 0, float("inf"))]
     assert autotune._pick(rep)[:3] == (4, 512, 512)
 
 
+def test_unroll_stage_times_the_fastest_shape_at_each_unroll():
+    """Stage 2 runs the stage-1 winner at U = 8, 4, 2; an explicit unroll is
+    kept only when it beats the default by more than the tie margin."""
+    cost = {(1, 256, 512): 10.0, (2, 256, 512): 8.0, (4, 512, 512): 9.0}
+    gain = {0: 1.0, 8: 1.0, 4: 0.95, 2: 0.90}
+    calls = []
+
+    def run(launch):
+        calls.append((launch.lanes_per_line, launch.block, launch.tile, launch.unroll))
+        return cost[(launch.lanes_per_line, launch.block, launch.tile)] * gain[launch.unroll]
+
+    best, report = autotune._tune(list(cost), run)
+    assert calls[:3] == [(1, 256, 512, 0), (2, 256, 512, 0), (4, 512, 512, 0)]
+    assert calls[3:] == [(2, 256, 512, 8), (2, 256, 512, 4), (2, 256, 512, 2)]
+    assert best[:4] == (2, 256, 512, 2) and len(report) == 6
+    gain.update({4: 1.0, 2: 1.005})
+    best, _ = autotune._tune(list(cost), run)
+    assert best[:4] == (2, 256, 512, 0)
+
+
+def test_old_cache_entries_read_as_default_unroll(tmp_path, monkeypatch):
+    monkeypatch.setenv("EMTRACE_TUNE_CACHE", str(tmp_path / "tune.json"))
+    autotune._store("k", [4, 512, 4096, 1.5])
+    assert autotune._lookup("k") == [4, 512, 4096, 0, 1.5]
+    launch = autotune._launch(autotune._lookup("k"))
+    assert (launch.lanes_per_line, launch.tile, launch.block, launch.unroll) == (4, 4096, 512, 0)
+
+
 def test_shipped_shapes_are_candidates_of_their_problem():
-    for key, (S, block, tile, _) in autotune.SHIPPED.items():
+    for key, (S, block, tile, U, _) in autotune.SHIPPED.items():
+        assert U in (0, 2, 4, 8)
         kind, arch, cus, *rest = key.split(":")
         assert arch == "gfx950" and cus == "256"
         if kind == "plain":

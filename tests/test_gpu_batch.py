@@ -104,15 +104,17 @@ def test_autotuner_picks_a_valid_shape(gpu, tmp_path, monkeypatch):
     c = torch.from_numpy(w.charges).cuda()
     s = torch.from_numpy(w.seeds).cuda()
     launch, report = autotune.tune_plain(c, s, rules.angle(), rows=16)
-    shape = (launch.lanes_per_line, launch.block, launch.tile)
-    assert shape in [r[:3] for r in report] and all(np.isfinite(r[3]) for r in report)
+    shape = (launch.lanes_per_line, launch.block, launch.tile, launch.unroll)
+    assert shape in [r[:4] for r in report] and all(np.isfinite(r[4]) for r in report)
+    assert {r[3] for r in report} == {0, 2, 4, 8}  # default, then each unroll
     again, rep2 = autotune.tune_plain(c, s, rules.angle(), rows=16)
-    assert (again.lanes_per_line, again.block, again.tile) == shape and rep2[0][-1] == "cached"
+    assert (again.lanes_per_line, again.block, again.tile, again.unroll) == shape
+    assert rep2[0][-1] == "cached"
     a = gpu.trace(c, s, w.len, rules.angle(), launch)
-    b = gpu.trace(c, s, w.len, rules.angle(), gpu.make_launch(shape[0], shape[2], shape[1]))
+    b = gpu.trace(c, s, w.len, rules.angle(), gpu.make_launch(shape[0], shape[2], shape[1], shape[3]))
     assert torch.equal(a, b)
     blaunch, brep = autotune.tune_batch(_configs(count=6, seed=2, length=40), rules.angle(), rows=8)
-    assert blaunch.lanes_per_line >= 1 and any(np.isfinite(r[3]) for r in brep)
+    assert blaunch.lanes_per_line >= 1 and any(np.isfinite(r[4]) for r in brep)
 
 
 def test_captured_checkpointed_graph_survives_plain_runs(gpu):
