@@ -217,9 +217,9 @@ def main():
 
 def profiled_value(name, key, workload, shape):
     """A per-launch figure from a committed counter profile, when this run used
-    the launch shape that was profiled (lanes 4, tile 4096, block 512)."""
+    the launch shape that was profiled (lanes 4, tile 4096, block 512, unroll 2)."""
     path = os.path.join(ROOT, "profiles", name)
-    if workload != "lines_64k" or shape != (4, 4096, 512) or not os.path.exists(path):
+    if workload != "lines_64k" or shape != (4, 4096, 512, 2) or not os.path.exists(path):
         return None
     for line in open(path):
         if f"{key}:" in line:
