@@ -13,8 +13,9 @@ problem size bucket and step rule (``$EMTRACE_TUNE_CACHE``, default
 Candidates are generated from the device's LDS budget: lanes per line S in
 {1, 2, ..., 32} (single configuration: until the grid holds 8 waves per SIMD;
 batch: 1-8), blocks of 256 and 512 threads, and tiles that hold all charges
-when they fit, else the largest tile that leaves room for one or two resident
-workgroups per CU.
+when they fit, else the largest tile that leaves room for one, two or four
+resident workgroups per CU (also eight for the batch's persistent grid, where
+more resident workgroups keep the SIMDs busy while items of uneven size end).
 """
 from __future__ import annotations
 
@@ -76,8 +77,10 @@ SHIPPED = {
     "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, 2, None],
     "plain:gfx950:256:16384:65536:2:1": [32, 512, 4096, 8, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule
-    # (1,000 rows: 451 ms; 256-thread blocks at unroll 8: 467 ms)
-    "batch:gfx950:256:512:8192:1048576:2:0": [4, 512, 4096, 2, None],
+    # (1,000 rows: 432 ms with 2,048-charge tiles, four workgroups per CU; 8
+    # lanes per line within 1 %; 4,096-charge tiles, two workgroups per CU:
+    # 452-467 ms; 8,192: 519 ms and more)
+    "batch:gfx950:256:512:8192:1048576:2:0": [4, 256, 2048, 4, None],
 }
 
 TIE = 1.01  # timings within 1% of the best count as ties
@@ -167,7 +170,7 @@ def candidates(p: int, n: int, cus: int, batch: bool = False,
             tiles = set()
             if stop and not batch:
                 tiles.update(t for t in (512, 1024, 2048, 4096) if t % q == 0 and t < need)
-            for resident in (1, 2):
+            for resident in ((1, 2, 4, 8) if batch else (1, 2, 4)):
                 tmax = (LDS_BYTES // resident // 16) // q * q
                 if tmax < q:
                     continue
