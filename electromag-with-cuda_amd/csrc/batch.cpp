@@ -169,6 +169,11 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
         while (S > 1 && max_p / (uint64_t)S < 64) S /= 2;
         req.lanes_per_line = S;
     }
+    // 256-thread blocks: with small tiles four workgroups share a CU, so while
+    // one re-stages a tile or finishes a short item the others keep issuing
+    // (512 configurations, up to 8,192 charges: 432 ms against 452-467 ms with
+    // 512 threads and tiles for two workgroups per CU)
+    if (!req.block) req.block = 256;
     Shape sh;
     int rc = choose_shape(dev, max_p, total_lines, &req, sh);
     if (rc) {
@@ -179,11 +184,12 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
         // residency of the persistent grid: as many blocks per CU as fit 4 waves/SIMD
         const int resident = std::max(1, 1024 / sh.block);
         const uint64_t Q = std::max<uint64_t>(256, (uint64_t)sh.S * kGroup);
-        uint64_t tmax = dev.lds_per_block / resident / sizeof(float4) / Q * Q;
+        uint64_t tmax = tile_budget(dev, resident) / Q * Q;
         tmax = std::max<uint64_t>(tmax, Q);
         const uint64_t need = (std::max<uint64_t>(max_p, 1) + Q - 1) / Q * Q;
         sh.tile = (uint32_t)std::min(need, tmax);
     }
+    if (!req.unroll) sh.unroll = default_unroll(dev, sh, max_p, total_lines, step->stop != 0);
     b->sh = sh;
     rc = persistent_grid(sh, dev, &b->grid);
     if (rc) {

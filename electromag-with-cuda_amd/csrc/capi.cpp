@@ -73,16 +73,16 @@ int validate_shape(const DeviceInfo& dev, const Shape& s)
     if (!pow2_in(s.block, 64, 512)) EM_FAIL(EM_ERR_INVALID, "block %d not in {64..512}", s.block);
     if (s.tile == 0 || s.tile % (uint32_t)(s.S * kGroup) || s.tile % 256)
         EM_FAIL(EM_ERR_INVALID, "tile %u not a positive multiple of 256 and of %d", s.tile, s.S * kGroup);
-    if ((size_t)s.tile * sizeof(float4) > dev.lds_per_block)
-        EM_FAIL(EM_ERR_INVALID, "tile %u x 16 B exceeds the %zu B of LDS per workgroup", s.tile,
-                dev.lds_per_block);
+    if ((uint64_t)s.tile > tile_budget(dev, 1))
+        EM_FAIL(EM_ERR_INVALID, "tile %u x 16 B + %zu B of static LDS exceeds the %zu B of LDS per "
+                "workgroup", s.tile, kLdsStatic, dev.lds_per_block);
     return EM_OK;
 }
 
 static uint32_t tile_for(const DeviceInfo& dev, uint64_t p, int S, int resident)
 {
     const uint64_t G = (uint64_t)S * kGroup;
-    const uint64_t budget = dev.lds_per_block / (uint64_t)std::max(1, resident) / sizeof(float4);
+    const uint64_t budget = tile_budget(dev, resident);
     const uint64_t Q = std::max<uint64_t>(256, G);  // tile granule
     uint64_t tmax = std::max<uint64_t>(Q, budget / Q * Q);
     const uint64_t need = (std::max<uint64_t>(p, 1) + Q - 1) / Q * Q;
@@ -93,15 +93,26 @@ static uint32_t tile_for(const DeviceInfo& dev, uint64_t p, int S, int resident)
 }
 
 // Charge records per pipelined group (the trace kernel takes 100 VGPRs at 8,
-// 68 at 4, 52 at 2; the rows do not depend on it).  Measured on MI355X: with
-// 512-thread blocks and no line ends, 2 is fastest (16,384 charges x 65,536
-// lines, 4 lanes per line: 831 ms against 917 ms at 8; the 512-configuration
-// batch: 451 against 464 ms); with 256-thread blocks or line ends, 8 is
-// fastest or within 2 % (the same batch at 256 threads: 467 against 519 ms at
-// 2; the 64-configuration checkpointed forward: 13.1 against 14.5 ms).
-static int default_unroll(const Shape& s, bool stop)
+// 68 at 4, 52 at 2; the rows do not depend on it).  Eight keep the most LDS
+// reads in flight per wave, which pays with few waves per SIMD and with line
+// ends.  With four or more resident waves per SIMD other waves hide the LDS
+// latency, and fewer records per group issue better: 2 when 512-thread blocks
+// re-stage the charges tile by tile, else 4.  Measured on MI355X (ms):
+//   16,384 charges x 65,536 lines, 4 lanes, 512 threads (re-staged): U=2 831, 4 847, 8 917
+//   the same, 256 threads, 2,304-charge tiles, 4 resident: U=4 best, 2 +4 %, 8 +2 %
+//   4,000 x 65,536, 512 threads (one tile): U=4 25.9, 2 27.3, 8 27.2
+//   512-configuration batch, 256 threads, 2,048-charge tiles: U=4 432, 2 448, 8 454;
+//     4,096-charge tiles (2 waves per SIMD): U=8 467, 4 482, 2 519
+//   64-configuration checkpointed forward with line ends: U=8 13.1, 2 14.5
+int default_unroll(const DeviceInfo& dev, const Shape& s, uint64_t p, uint64_t n, bool stop)
 {
-    return (s.block >= 512 && !stop) ? 2 : 8;
+    if (stop) return 8;
+    const uint64_t blocks = (n * (uint64_t)s.S + s.block - 1) / s.block;
+    const uint64_t by_lds = dev.lds_per_block / ((uint64_t)s.tile * sizeof(float4) + kLdsStatic);
+    const uint64_t per_cu = std::min<uint64_t>((blocks + dev.cus - 1) / dev.cus, std::max<uint64_t>(1, by_lds));
+    const uint64_t waves_per_simd = per_cu * (uint64_t)s.block / 256;
+    if (waves_per_simd < 4) return 8;
+    return (s.block >= 512 && p > s.tile) ? 2 : 4;
 }
 
 int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
@@ -120,7 +131,7 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
         while (block > 64 && n * (uint64_t)s.S / (uint64_t)block < (uint64_t)dev.cus) block /= 2;
         s.block = (req && req->block) ? req->block : block;
         s.tile = (req && req->tile) ? (uint32_t)req->tile : tile_for(dev, p, s.S, 2);
-        s.unroll = (req && req->unroll) ? req->unroll : default_unroll(s, true);
+        s.unroll = (req && req->unroll) ? req->unroll : default_unroll(dev, s, p, n, true);
         EM_TRY(validate_shape(dev, s));
         out = s;
         return EM_OK;
@@ -128,8 +139,7 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
     // With line ends a wave keeps evaluating until its last line has ended (a
     // whole block, when the charges are re-staged tile by tile): few lines per
     // wave, and one-wave blocks over small tiles when the charges do not fit.
-    const bool restaged = (uint64_t)p * sizeof(float4) > dev.lds_per_block;
-    s.block = (req && req->block) ? req->block : (stop && restaged ? 64 : 256);
+    const bool restaged = (uint64_t)p > tile_budget(dev, 1);
     if (req && req->lanes_per_line) {
         s.S = req->lanes_per_line;
     } else if (stop) {
@@ -137,11 +147,33 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
         while (S < 32 && p / (2ull * S) >= 64) S *= 2;
         s.S = S;
     } else {
-        const uint64_t target_waves = 2ull * 4ull * (uint64_t)dev.cus;  // two waves per SIMD
+        // Split lines over more lanes while the grid is short of waves, as long
+        // as each lane keeps enough charges to amortize the per-step reduction
+        // and step rule: >= 64 charges per lane up to 4 waves per SIMD, >= 32 up
+        // to 2, >= 16 up to 1.  Measured on MI355X (1,000 rows, default block):
+        // this picks the fastest S, or one within 4 %, for p x n = 300 x 8,192,
+        // 1,000 x 8,192, 1,000 x 2,048, 2,000 x 16,384, 600 x 30,000,
+        // 150 x 65,536, 5,000 x 1,024 and 16,384 x 65,536.
+        const uint64_t simds = 4ull * (uint64_t)dev.cus;
+        auto split = [&](int S) {
+            const uint64_t waves = (n * (uint64_t)S + 63) / 64;
+            const uint64_t per_lane = p / (2ull * (uint64_t)S);  // after doubling S
+            return (waves < 4 * simds && per_lane >= 64) || (waves < 2 * simds && per_lane >= 32) ||
+                   (waves < simds && per_lane >= 16);
+        };
         int S = 1;
-        while (S < 32 && (n * S + 63) / 64 < target_waves && p / (2ull * S) >= 64) S *= 2;
+        while (S < 32 && split(S)) S *= 2;
         s.S = S;
     }
+    // 512-thread blocks once the grid holds two of them per CU (four waves per
+    // SIMD): 12 % faster than 256 threads on 16,384 x 65,536 and 8,000 x
+    // 32,768, within 2 % on one-tile problems.
+    if (req && req->block)
+        s.block = req->block;
+    else if (stop)
+        s.block = restaged ? 64 : 256;
+    else
+        s.block = n * (uint64_t)s.S >= 2ull * 512ull * (uint64_t)dev.cus ? 512 : 256;
     if (req && req->tile) {
         s.tile = (uint32_t)req->tile;
     } else if (stop && restaged) {
@@ -152,7 +184,7 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
                                                       std::max(1, 1024 / s.block));
         s.tile = tile_for(dev, p, s.S, resident);
     }
-    s.unroll = (req && req->unroll) ? req->unroll : default_unroll(s, stop);
+    s.unroll = (req && req->unroll) ? req->unroll : default_unroll(dev, s, p, n, stop);
     EM_TRY(validate_shape(dev, s));
     out = s;
     return EM_OK;

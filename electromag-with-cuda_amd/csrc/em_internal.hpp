@@ -48,6 +48,18 @@ struct DeviceInfo {
 };
 int current_device_info(DeviceInfo& out);  // inits lazily; EM_ERR_NO_DEVICE if none
 
+// LDS a workgroup needs beside its charge tile: the tracer's kernels declare
+// 256-272 B of static LDS, and static plus dynamic LDS must fit the
+// workgroup's (and, for `resident` workgroups per CU, the CU's) budget.
+constexpr size_t kLdsStatic = 512;
+
+// Charge records that fit one tile when `resident` workgroups share a CU.
+inline uint64_t tile_budget(const DeviceInfo& dev, int resident)
+{
+    const size_t share = dev.lds_per_block / (size_t)(resident < 1 ? 1 : resident);
+    return share > kLdsStatic ? (share - kLdsStatic) / sizeof(float4) : 0;
+}
+
 // ---- launch shape ----------------------------------------------------------
 struct Shape {
     int S = 1;         // lanes per line
@@ -65,6 +77,9 @@ struct Shape {
 int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
                  bool stop = false, bool refill = false);
 int validate_shape(const DeviceInfo& dev, const Shape& s);
+// The unroll a shape gets when the request leaves it 0 (from the resident
+// waves per SIMD, whether the charges are re-staged, and line ends).
+int default_unroll(const DeviceInfo& dev, const Shape& s, uint64_t p, uint64_t n, bool stop);
 
 StepP to_stepp(const em_step& st);
 int validate_step(const em_step* st);

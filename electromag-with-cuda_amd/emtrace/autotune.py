@@ -28,6 +28,7 @@ from . import api
 from .rules import StepRule
 
 LDS_BYTES = 160 * 1024
+LDS_STATIC = 512  # static LDS of the trace kernels (256-272 B), kept free beside the tile
 GROUP = 16  # charges per lane per pipelined pair: tiles are multiples of S * GROUP
 
 
@@ -171,7 +172,7 @@ def candidates(p: int, n: int, cus: int, batch: bool = False,
             if stop and not batch:
                 tiles.update(t for t in (512, 1024, 2048, 4096) if t % q == 0 and t < need)
             for resident in ((1, 2, 4, 8) if batch else (1, 2, 4)):
-                tmax = (LDS_BYTES // resident // 16) // q * q
+                tmax = ((LDS_BYTES // resident - LDS_STATIC) // 16) // q * q
                 if tmax < q:
                     continue
                 if need <= tmax:

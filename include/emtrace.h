@@ -55,11 +55,13 @@ typedef struct em_step {
                          remaining rows repeat the end point.  0 = always len-1 steps. */
 } em_step;
 
-/* Launch shape.  Any field left 0 is chosen by the library (autotuner cache,
- * else a heuristic from the problem size and the device's CU count). */
+/* Launch shape.  Any field left 0 is chosen by the library from the problem
+ * size and the device's CU count (the Python autotuner passes all four). */
 typedef struct em_launch {
     int32_t lanes_per_line; /* 1, 2, 4, 8, 16 or 32 lanes split one line's charge sum */
-    int32_t tile;           /* charges per LDS tile (multiple of 64)                  */
+    int32_t tile;           /* charges per LDS tile: a multiple of 256 and of
+                               16 x lanes_per_line; tile x 16 B plus 512 B of static
+                               LDS must fit a workgroup (at most 9,984 on gfx950)   */
     int32_t block;          /* threads per workgroup: 64, 128, 256 or 512             */
     int32_t unroll;         /* charge records per lane per pipelined group: 2, 4 or 8;
                                any value gives the same bits (speed only)             */

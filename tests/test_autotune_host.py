@@ -13,7 +13,7 @@ def test_candidates_fit_lds_and_granules():
         for S, block, tile in cands:
             assert S in (1, 2, 4, 8, 16, 32) and block in (256, 512)
             assert tile % max(256, S * autotune.GROUP) == 0
-            assert tile * 16 <= autotune.LDS_BYTES
+            assert tile * 16 + autotune.LDS_STATIC <= autotune.LDS_BYTES
             if batch:
                 assert S <= 8
 

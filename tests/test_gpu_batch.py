@@ -32,6 +32,26 @@ def test_batched_equals_single_configuration(gpu, use_graph):
     assert abs(info["interactions"] - layout.interactions()) < 1
 
 
+def test_batched_default_shape_with_a_large_configuration(gpu):
+    """A configuration of 10,000 charges next to small ones: the default tile
+    (sized for four workgroups per CU beside the kernels' static LDS) re-stages
+    its charges every step; rows bitwise equal to the single-configuration
+    kernel with the plan's shape."""
+    ws = _configs(count=3, seed=12, length=30) + [gen.plain_config(10000, 100, 10.0, 30, seed=13)]
+    layout, charges, lines = gpu.pack_batch(ws)
+    plan = gpu.BatchPlan(layout, rules.angle(stop=False), None, use_graph=False)
+    info = plan.info()
+    assert info["tile"] < 10000 and info["tile"] * 16 + 512 <= 160 * 1024
+    plan.run(charges, lines)
+    torch.cuda.synchronize()
+    launch = gpu.make_launch(info["lanes_per_line"], info["tile"], info["block"])
+    for i, w in enumerate(ws):
+        single = gpu.trace(torch.from_numpy(w.charges).cuda(), torch.from_numpy(w.seeds).cuda(),
+                           w.len, rules.angle(stop=False), launch)
+        assert torch.equal(layout.rows_view(lines, i), single), i
+    plan.close()
+
+
 def test_graph_replays_match_eager(gpu):
     ws = _configs(count=9, seed=4, length=80)
     layout, charges, lines = gpu.pack_batch(ws)

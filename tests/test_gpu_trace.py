@@ -227,6 +227,11 @@ EDGE = [
     ("p_tile_plus_one", 257, 96, 80, rules.plain(10.0), (2, 256, 256)),
     ("p_tile_minus_one", 255, 96, 80, rules.angle(), (2, 256, 256)),
     ("wide_groups_tiny_p", 40, 64, 60, rules.angle(), (32, 512, 64)),
+    # the largest charge sets one LDS tile holds beside the kernels' static LDS
+    # (9,984 records + 512 B within 160 KiB), and just past it
+    ("p_largest_single_tile", 9984, 64, 40, rules.plain(10.0), None),
+    ("p_past_single_tile", 10000, 64, 40, rules.angle(stop=False), None),
+    ("p_10240", 10240, 64, 40, rules.plain(10.0), None),
 ]
 
 
@@ -254,6 +259,18 @@ def test_edge_shapes_vs_cpu_path(gpu, orc, case):
     print(f"[edge {name}] lines {n}: median {np.median(err):.2e} max {err.max():.2e}", flush=True)
     assert np.isfinite(rows).all()
     assert (err <= 1e-4).mean() >= 0.95 and np.median(err) <= 1e-5
+
+
+def test_tile_beyond_the_lds_budget_is_rejected(gpu):
+    """A tile must leave room for the kernels' static LDS: 10,240 records
+    (exactly 160 KiB) is refused before any launch; 9,984 runs."""
+    from emtrace._lib import EmError
+    w = gen.plain_config(10240, 64, 5.0, 8, seed=3)
+    with pytest.raises(EmError) as e:
+        gpu_trace(gpu, w.charges, w.seeds, w.len, rules.plain(10.0), gpu.make_launch(1, 10240, 256))
+    assert "static LDS" in str(e.value)
+    rows = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.plain(10.0), gpu.make_launch(1, 9984, 256))
+    assert np.isfinite(rows).all()
 
 
 def test_no_charges_gives_the_reference_nan(gpu, orc):
