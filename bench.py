@@ -349,8 +349,8 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
         res["fp64_kernel"] = {"interactions_per_s": i64 / min(k64), "kernel_ms": min(k64) * 1e3,
                               "rows": rows_n, "step_rule": "plain",
                               "frac_fp64_peak": i64 / min(k64) * FLOP_PER_INTERACTION / 78.6e12,
-                              "note": "one lane per line, charges summed in index order; correctly "
-                                      "rounded sqrt and 3 divisions per interaction"}
+                              "note": "one lane per line, charges summed in index order; a correctly "
+                                      "rounded sqrt and division per interaction (plain rule)"}
         del r64, q64, s64
     # 6. the reference application's default problem (ElectroMag.cpp:43: 1,024
     #    charges, 128 x 128 grid seeds, 2,500 rows, curvature step, resolution 1)

@@ -59,18 +59,24 @@ __device__ __forceinline__ void term64(const double4 c, double px, double py, do
     }
 }
 
+// Charges whose terms are in flight together.  With one wave per SIMD (65,536
+// lines) 4 is fastest: the plain rule's 250-row trace of 16,384 charges takes
+// 440 ms, against 489 ms at 2, 497 ms at 8 and 564 ms without unrolling.
+constexpr int kUnroll64 = 4;
+
 template <int RULE>
 __device__ __forceinline__ void sum_range(const double4* __restrict__ tile, uint32_t count,
                                           double px, double py, double pz, double& sx, double& sy,
                                           double& sz)
 {
     uint32_t i = 0;
-    for (; i + 4 <= count; i += 4) {
-        double e[4][3];
+    for (; i + kUnroll64 <= count; i += kUnroll64) {
+        double e[kUnroll64][3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) term64<RULE>(tile[i + u], px, py, pz, e[u][0], e[u][1], e[u][2]);
+        for (int u = 0; u < kUnroll64; ++u)
+            term64<RULE>(tile[i + u], px, py, pz, e[u][0], e[u][1], e[u][2]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kUnroll64; ++u) {
             sx += e[u][0];
             sy += e[u][1];
             sz += e[u][2];
