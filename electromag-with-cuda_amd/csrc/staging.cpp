@@ -13,15 +13,6 @@
 // buffer with chunk c+1, a small team of host threads copies chunk c out of
 // the other into the caller's arrays.  Host-to-device copies use the same
 // buffers the other way round.
-//
-// Device-to-pinned copies run as a small kernel of this file (kCopyBlocks
-// workgroups writing straight into the mapped staging buffer, 44 GB/s) rather
-// than through hipMemcpyAsync, whose device-to-host copies ran as the
-// runtime's own copy kernel.  Either copy slows the tracer's row chunk that it
-// overlaps; measured on the reference's default problem (1,024 charges,
-// 16,384 lines, 625-row chunks of 4.4 ms alone): 6.4-6.6 ms per chunk beside
-// the runtime's copies, 5.8-5.9 ms beside this kernel (8 to 64 workgroups
-// alike), and 27.6 against 25.3 ms per drop-in call.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,44 +28,14 @@ namespace em {
 namespace {
 
 constexpr size_t kStageChunk = 64ull << 20;  // bytes per staging buffer
-constexpr int kCopyBlocks = 32;
-constexpr int kCopyThreads = 256;
 
 struct Staging {
     std::mutex mu;
     void* pin[2] = {nullptr, nullptr};
-    void* dpin[2] = {nullptr, nullptr};  // the same buffers as device addresses
     size_t cap = 0;  // bytes of each buffer
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 Staging g_stage[64];
-
-template <class T>
-__global__ void __launch_bounds__(kCopyThreads) k_to_pinned(const T* __restrict__ src,
-                                                             T* __restrict__ dst, size_t count)
-{
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
-        dst[i] = src[i];
-}
-
-// Device bytes [src, src + bytes) into the pinned buffer at device address dst.
-int copy_to_pinned(void* dst, const void* src, size_t bytes, hipStream_t stream)
-{
-    const uintptr_t a = reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst) | bytes;
-    if (a % 16 == 0) {
-        hipLaunchKernelGGL(k_to_pinned<uint4>, dim3(kCopyBlocks), dim3(kCopyThreads), 0, stream,
-                           static_cast<const uint4*>(src), static_cast<uint4*>(dst), bytes / 16);
-    } else if (a % 4 == 0) {
-        hipLaunchKernelGGL(k_to_pinned<uint32_t>, dim3(kCopyBlocks), dim3(kCopyThreads), 0, stream,
-                           static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), bytes / 4);
-    } else {
-        EM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
-        return EM_OK;
-    }
-    EM_HIP(hipGetLastError());
-    return EM_OK;
-}
 
 // Buffers of at least min(largest segment, kStageChunk) bytes (pinned memory
 // is slow to allocate: small transfers keep small buffers).
@@ -86,13 +47,10 @@ int ensure_staging(Staging& st, const CopySeg* segs, int nseg)
     if (st.cap < need) {
         for (int i = 0; i < 2; ++i) {
             if (st.pin[i]) EM_HIP(hipHostFree(st.pin[i]));
-            st.pin[i] = st.dpin[i] = nullptr;
+            st.pin[i] = nullptr;
         }
         st.cap = 0;
-        for (int i = 0; i < 2; ++i) {
-            EM_HIP(hipHostMalloc(&st.pin[i], need, hipHostMallocDefault));
-            EM_HIP(hipHostGetDevicePointer(&st.dpin[i], st.pin[i], 0));
-        }
+        for (int i = 0; i < 2; ++i) EM_HIP(hipHostMalloc(&st.pin[i], need, hipHostMallocDefault));
         st.cap = need;
     }
     for (int i = 0; i < 2; ++i)
@@ -129,7 +87,7 @@ int d2h_staged(int device, const CopySeg* segs, int nseg, hipStream_t stream)
                                std::min(st.cap, segs[i].bytes - off)});
     if (ch.empty()) return EM_OK;
     auto issue = [&](size_t c) -> int {
-        EM_TRY(copy_to_pinned(st.dpin[c & 1], ch[c].src, ch[c].bytes, stream));
+        EM_HIP(hipMemcpyAsync(st.pin[c & 1], ch[c].src, ch[c].bytes, hipMemcpyDeviceToHost, stream));
         EM_HIP(hipEventRecord(st.ev[c & 1], stream));
         return EM_OK;
     };
@@ -235,7 +193,7 @@ int staging_release_all()
         for (int i = 0; i < 2; ++i) {
             if (st.pin[i] && hipHostFree(st.pin[i]) != hipSuccess) rc = EM_ERR_HIP;
             if (st.ev[i] && hipEventDestroy(st.ev[i]) != hipSuccess) rc = EM_ERR_HIP;
-            st.pin[i] = st.dpin[i] = nullptr;
+            st.pin[i] = nullptr;
             st.ev[i] = nullptr;
         }
         st.cap = 0;
