@@ -18,8 +18,11 @@
 // Device layout: particles are 8-float records {x, y, z, q, vx, vy, vz, m}
 // (electro::dynamicPointCharge<float>: pointCharge, velocity, mass); static
 // charges are {x, y, z, q}; accelerations are n x 3 floats (Array<Vector3>).
-// One lane per particle.  Static charges go through the tracer's pipelined
-// tile walk (FieldBody).  Dynamic sources are staged as two float4 records,
+// S lanes per particle (S = 1..8: until the grid holds four waves per SIMD;
+// 65,536 particles among 16,384 charges: 3.15-3.23 ms at S = 4 against
+// 3.81-3.89 ms at S = 1, where one wave per SIMD leaves every latency
+// exposed).  Static charges go through the tracer's pipelined tile walk
+// (FieldBody).  Dynamic sources are staged as two float4 records,
 // {x, y, z, q} and {1e-7 q v, index}; the index zeroes the self term.
 // 21 VALU + 2 for the self mask + 1 transcendental per particle pair.
 #include <hip/hip_runtime.h>
@@ -27,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "em_internal.hpp"
 
@@ -35,6 +39,8 @@ namespace {
 
 constexpr int kNbBlock = 256;
 constexpr uint32_t kNbTile = 4096;          // float4 records per LDS tile (64 KiB)
+constexpr uint32_t kNbTileSplit = 2048;     // with S > 1: 32 KiB, four workgroups per CU
+constexpr int kNbMaxLanes = 8;
 constexpr float kMagneto = 1e-7f;           // mu_0 / (4 pi) (Magnetics.h magneto_k)
 constexpr float kCoulombF = 8.987551787E9f; // include/Electrostatics.h:27
 
@@ -104,14 +110,19 @@ __device__ __forceinline__ void stage_dyn(float4* __restrict__ tile,
     }
 }
 
+// S lanes per particle (lane k sums records k, k + S, ... of every tile, an
+// xor butterfly combines the S partial sums); `tile` float4 records of LDS.
+template <int S, int U>
 __global__ __launch_bounds__(kNbBlock) void k_verlet_step(const float4* __restrict__ parts,
                                                           uint32_t n,
                                                           const float4* __restrict__ charges,
                                                           uint32_t p, float* __restrict__ accel,
-                                                          VerletP vp, float4* __restrict__ out)
+                                                          VerletP vp, float4* __restrict__ out,
+                                                          uint32_t tile)
 {
     extern __shared__ float4 nb_tile[];
-    const uint32_t i = blockIdx.x * kNbBlock + threadIdx.x;
+    const uint32_t i = (blockIdx.x * kNbBlock + threadIdx.x) / S;
+    const int k = threadIdx.x % S;
     const bool valid = i < n;
     float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = make_float4(0.f, 0.f, 0.f, 1.f);
     float ax = 0.f, ay = 0.f, az = 0.f;
@@ -127,32 +138,38 @@ __global__ __launch_bounds__(kNbBlock) void k_verlet_step(const float4* __restri
     float x = c0.x + vx * vp.dt, y = c0.y + vy * vp.dt, z = c0.z + vz * vp.dt;
 
     // static charges
-    constexpr uint32_t G = kGroup;  // S = 1
+    constexpr uint32_t G = S * kGroup;
     FieldBody fs;
     fs.init(x, y, z);
-    for (uint32_t t0 = 0; t0 < p; t0 += kNbTile) {
-        const uint32_t cnt = min(kNbTile, p - t0), cntp = round_up(cnt, G);
+    for (uint32_t t0 = 0; t0 < p; t0 += tile) {
+        const uint32_t cnt = min(tile, p - t0), cntp = round_up(cnt, G);
         __syncthreads();
         stage(nb_tile, charges + t0, cnt, cntp);
         __syncthreads();
-        pipelined_tile<1, 1>(nb_tile, cntp, 0, fs);
+        pipelined_tile<S, 1, U>(nb_tile, cntp, k, fs);
     }
     // other particles (old positions and velocities)
     DynBody fd;
     fd.init(x, y, z, valid ? (int)i : -2);
-    constexpr uint32_t kDynTile = kNbTile / 2;
-    for (uint32_t t0 = 0; t0 < n; t0 += kDynTile) {
-        const uint32_t cnt = min(kDynTile, n - t0), cntp = round_up(cnt, G);
+    const uint32_t dyn_tile = tile / 2;
+    for (uint32_t t0 = 0; t0 < n; t0 += dyn_tile) {
+        const uint32_t cnt = min(dyn_tile, n - t0), cntp = round_up(cnt, G);
         __syncthreads();
         stage_dyn(nb_tile, parts, t0, cnt, cntp);
         __syncthreads();
-        pipelined_tile<1, 2>(nb_tile, cntp, 0, fd);
+        pipelined_tile<S, 2, U>(nb_tile, cntp, k, fd);
     }
-    if (!valid) return;
-    const float Ex = kCoulombF * ((fs.acc.sx + fs.acc.cx) + (fd.e.sx + fd.e.cx));
-    const float Ey = kCoulombF * ((fs.acc.sy + fs.acc.cy) + (fd.e.sy + fd.e.cy));
-    const float Ez = kCoulombF * ((fs.acc.sz + fs.acc.cz) + (fd.e.sz + fd.e.cz));
-    const float Bx = fd.b.sx + fd.b.cx, By = fd.b.sy + fd.b.cy, Bz = fd.b.sz + fd.b.cz;
+    // every lane of the group ends with the same bits; lane 0 writes
+    const float Esx = group_sum<S>(fs.acc.sx + fs.acc.cx), Edx = group_sum<S>(fd.e.sx + fd.e.cx);
+    const float Esy = group_sum<S>(fs.acc.sy + fs.acc.cy), Edy = group_sum<S>(fd.e.sy + fd.e.cy);
+    const float Esz = group_sum<S>(fs.acc.sz + fs.acc.cz), Edz = group_sum<S>(fd.e.sz + fd.e.cz);
+    const float Bx = group_sum<S>(fd.b.sx + fd.b.cx);
+    const float By = group_sum<S>(fd.b.sy + fd.b.cy);
+    const float Bz = group_sum<S>(fd.b.sz + fd.b.cz);
+    if (!valid || k != 0) return;
+    const float Ex = kCoulombF * (Esx + Edx);
+    const float Ey = kCoulombF * (Esy + Edy);
+    const float Ez = kCoulombF * (Esz + Edz);
     // a = q E + q (v x B), then / m  (Particle_System.cpp:173-178)
     const float q = c0.w;
     ax = q * Ex + q * (vy * Bz - vz * By);
@@ -178,16 +195,42 @@ __global__ __launch_bounds__(kNbBlock) void k_verlet_step(const float4* __restri
     }
 }
 
+template <int S, int U>
+int launch_verlet_s(const float* parts, uint32_t n, const float* charges, uint32_t p, float* accel,
+                    const VerletP& vp, float* out, uint32_t tile, hipStream_t stream)
+{
+    const uint32_t ppb = kNbBlock / S;  // particles per workgroup
+    const dim3 grid((n + ppb - 1) / ppb), block(kNbBlock);
+    hipLaunchKernelGGL((k_verlet_step<S, U>), grid, block, tile * sizeof(float4), stream,
+                       reinterpret_cast<const float4*>(parts), n,
+                       reinterpret_cast<const float4*>(charges), p, accel, vp,
+                       reinterpret_cast<float4*>(out), tile);
+    EM_HIP(hipGetLastError());
+    return EM_OK;
+}
+
 int launch_verlet(const float* parts, uint32_t n, const float* charges, uint32_t p, float* accel,
                   const VerletP& vp, float* out, hipStream_t stream)
 {
-    const dim3 grid((n + kNbBlock - 1) / kNbBlock), block(kNbBlock);
-    hipLaunchKernelGGL(k_verlet_step, grid, block, kNbTile * sizeof(float4), stream,
-                       reinterpret_cast<const float4*>(parts), n,
-                       reinterpret_cast<const float4*>(charges), p, accel, vp,
-                       reinterpret_cast<float4*>(out));
-    EM_HIP(hipGetLastError());
-    return EM_OK;
+    DeviceInfo dev;
+    EM_TRY(current_device_info(dev));
+    // lanes per particle: until the grid holds four waves per SIMD, unless
+    // EMTRACE_VERLET_LANES (1, 2, 4 or 8) names a number
+    int S = 1;
+    while (S < kNbMaxLanes && (uint64_t)n * S < 4ull * 4ull * 64ull * (uint64_t)dev.cus) S *= 2;
+    if (const char* env = std::getenv("EMTRACE_VERLET_LANES")) {
+        const int want = std::atoi(env);
+        if (want != 1 && want != 2 && want != 4 && want != 8)
+            EM_FAIL(EM_ERR_INVALID, "EMTRACE_VERLET_LANES=%s not in {1, 2, 4, 8}", env);
+        S = want;
+    }
+    const uint32_t tile = S == 1 ? kNbTile : kNbTileSplit;
+    switch (S) {
+        case 1: return launch_verlet_s<1, 8>(parts, n, charges, p, accel, vp, out, tile, stream);
+        case 2: return launch_verlet_s<2, 4>(parts, n, charges, p, accel, vp, out, tile, stream);
+        case 4: return launch_verlet_s<4, 4>(parts, n, charges, p, accel, vp, out, tile, stream);
+        default: return launch_verlet_s<8, 4>(parts, n, charges, p, accel, vp, out, tile, stream);
+    }
 }
 
 int check_verlet(size_t n, size_t p, const float* box_min, const float* box_max, float dt)

@@ -238,7 +238,12 @@ int em_batch_adjoint(em_batch* plan, const float* d_charges, const float* d_x,
  * alias the input) the step also writes x' and v' + a dt/2, each reflected
  * into [box_min, box_max] as BoundToBox does (Particle_System.cpp:19-56).
  * box_min / box_max are host float[3].  Status: 1 when n == 0 (the
- * reference's only check), 16+ as elsewhere. */
+ * reference's only check), 16+ as elsewhere.
+ * Each particle's sum over the sources is split over 1-8 lanes (as many as
+ * bring the grid to four waves per SIMD); the environment variable
+ * EMTRACE_VERLET_LANES = 1, 2, 4 or 8 overrides that count (any other value:
+ * EM_ERR_INVALID).  The split changes the order of the FP32 sums, so the
+ * last bits of a may differ between counts. */
 int em_verlet_step_f32(float* particles, size_t n, const float* charges,
                        size_t p, float* accel, const float* box_min,
                        const float* box_max, float dt, int write_back,

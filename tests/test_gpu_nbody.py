@@ -67,6 +67,29 @@ def test_accelerations_vs_fp64(gpu, orc, case):
     assert np.abs(Pg[:, 4:7] - out64[:, 4:7]).max() <= vtol
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_lanes_per_particle_vs_fp64(gpu, orc, monkeypatch, lanes):
+    """The sum over sources split over 1-8 lanes per particle (the library
+    picks the count from the grid size; EMTRACE_VERLET_LANES overrides it):
+    every split meets the FP64 bar, single- and multi-tile."""
+    n, p = 3000, 5000
+    P, ch, a = _system(n, p, seed=77)
+    dt = 1e-3
+    a64, _ = orc.verlet64(P, ch, a, [-6] * 3, [6] * 3, dt, write_back=False)
+    monkeypatch.setenv("EMTRACE_VERLET_LANES", str(lanes))
+    acc = a.copy()
+    gpu.verlet_step(P.copy(), ch, acc, [-6] * 3, [6] * 3, dt, write_back=False)
+    _check_acc(acc, a64, f"lanes {lanes}")
+
+
+def test_bad_lanes_override_is_rejected(gpu, monkeypatch):
+    from emtrace._lib import EmError
+    P, ch, a = _system(64, 10, seed=5)
+    monkeypatch.setenv("EMTRACE_VERLET_LANES", "3")
+    with pytest.raises(EmError):
+        gpu.verlet_step(P.copy(), ch, a.copy(), [-6] * 3, [6] * 3, 1e-3, write_back=False)
+
+
 def test_reference_semantics_and_device_form(gpu, orc):
     P, ch, a = _system(700, 300, seed=9)
     acc_h = a.copy()
