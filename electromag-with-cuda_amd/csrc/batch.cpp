@@ -162,11 +162,22 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
     em_launch req{};
     if (launch) req = *launch;
     if (!req.lanes_per_line) {
-        // 4 lanes per line: 64-line work items even out the persistent grid's
-        // finishing times (measured on 512 uneven configurations: S = 4 ran
-        // 1.3x faster than S = 1), as long as every lane keeps >= 64 charges.
         int S = 4;
-        while (S > 1 && max_p / (uint64_t)S < 64) S /= 2;
+        if (step->stop) {
+            // line ends: a wave runs until its longest line has ended, so
+            // wide groups (up to 16 lanes, >= 16 charges each); the 512
+            // configurations with line ends: S=16 113 ms, S=32 119, S=4 199;
+            // 64 configurations of <= 1,024 charges, checkpointed forward +
+            // adjoint: S=16 25.0 ms, S=4 39.8
+            S = 1;
+            while (S < 16 && max_p / (2ull * S) >= 16) S *= 2;
+        } else {
+            // 4 lanes per line: 64-line work items even out the persistent
+            // grid's finishing times (measured on 512 uneven configurations:
+            // S = 4 ran 1.3x faster than S = 1), as long as every lane keeps
+            // >= 64 charges
+            while (S > 1 && max_p / (uint64_t)S < 64) S /= 2;
+        }
         req.lanes_per_line = S;
     }
     // 256-thread blocks: with small tiles four workgroups share a CU, so while

@@ -143,8 +143,14 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
     if (req && req->lanes_per_line) {
         s.S = req->lanes_per_line;
     } else if (stop) {
+        // With line ends a wave runs until the longest of its 64/S lines has
+        // ended, so wide groups pay even at few charges per lane: S doubles
+        // while each lane keeps >= 16 charges.  Measured (checkpointed
+        // forward, 500 rows, ms): p = 128: S=8 0.74 (S=2 1.37); p = 256 x 2,048
+        // lines: S=16 0.75 (S=4 1.37); p = 1,024: S=32 1.03 (S=16 1.45);
+        // p = 2,000 x 65,536 lines: S=32 5.14 (S=16 6.60).
         int S = 1;
-        while (S < 32 && p / (2ull * S) >= 64) S *= 2;
+        while (S < 32 && p / (2ull * S) >= 16) S *= 2;
         s.S = S;
     } else {
         // Split lines over more lanes while the grid is short of waves, as long
