@@ -595,6 +595,12 @@ struct AdjPlan {
     size_t w_plane = 0;
     float4* d_part = nullptr;
     size_t part_bytes = 0;
+    // single-configuration calls: the descriptor the work lists were built
+    // for, and the stream and completion event of the last call's launches
+    Cfg single{};
+    bool single_ok = false;
+    hipStream_t last = nullptr;
+    hipEvent_t done = nullptr;
 };
 
 static int grow(void** p, size_t* have, size_t need)
@@ -615,6 +621,7 @@ int adj_free(AdjPlan* a)
                   a->d_rec, a->d_w, a->d_part};
     for (void* p : ps)
         if (p) EM_HIP(hipFree(p));
+    if (a->done) EM_HIP(hipEventDestroy(a->done));
     *a = AdjPlan();
     return EM_OK;
 }
@@ -770,27 +777,45 @@ extern "C" int em_adjoint_dev_f32(const float* d_charges, size_t p, const float*
     Shape sh;
     EM_TRY(choose_shape(dev, p, n, launch, sh, step->stop != 0));  // as the forward chose
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!p) {
+        if (d_grad_seeds) EM_HIP(hipMemsetAsync(d_grad_seeds, 0, 3 * n * sizeof(float), st));
+        return EM_OK;
+    }
     std::lock_guard<std::mutex> lk(g_adj_mu);
     AdjPlan* a = &g_adj[dev.device];
-    // earlier calls on this stream may still use the shared work buffers
-    EM_HIP(hipStreamSynchronize(st));
     Cfg cfg{};
     cfg.pitch = pitch;
     cfg.p = (uint32_t)p;
     cfg.n = (uint32_t)n;
     cfg.len = (uint32_t)len;
     cfg.grad_line_off = (uint32_t)n;  // seed-gradient plane stride
-    std::vector<Cfg> cfgs{cfg};
-    EM_TRY(adj_prepare(a, cfgs, sh, ckpt_every, dev));
-    if (!a->d_cfgs_own) EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_cfgs_own), sizeof(Cfg)));
-    EM_HIP(hipMemcpy(a->d_cfgs_own, &cfg, sizeof(Cfg), hipMemcpyHostToDevice));
-    if (!p) {
-        if (d_grad_seeds) EM_HIP(hipMemsetAsync(d_grad_seeds, 0, 3 * n * sizeof(float), st));
-        return EM_OK;
+    // The work lists and buffers are rebuilt only for a new problem or shape;
+    // a repeated call reuses them without host synchronization (on another
+    // stream it first waits, on the device, for the last call's launches).
+    const bool same = a->single_ok && a->device == dev.device && a->every == ckpt_every &&
+                      a->sh.S == sh.S && a->sh.block == sh.block && a->sh.tile == sh.tile &&
+                      a->sh.unroll == sh.unroll && a->single.pitch == cfg.pitch &&
+                      a->single.p == cfg.p && a->single.n == cfg.n && a->single.len == cfg.len;
+    if (!same) {
+        // earlier calls, on any stream, may still use the shared work buffers
+        if (a->done) EM_HIP(hipEventSynchronize(a->done));
+        a->single_ok = false;
+        std::vector<Cfg> cfgs{cfg};
+        EM_TRY(adj_prepare(a, cfgs, sh, ckpt_every, dev));
+        if (!a->d_cfgs_own) EM_HIP(hipMalloc(reinterpret_cast<void**>(&a->d_cfgs_own), sizeof(Cfg)));
+        EM_HIP(hipMemcpy(a->d_cfgs_own, &cfg, sizeof(Cfg), hipMemcpyHostToDevice));
+        if (!a->done) EM_HIP(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
+        a->single = cfg;
+        a->single_ok = true;
+    } else if (a->last != st) {
+        EM_HIP(hipStreamWaitEvent(st, a->done, 0));
     }
-    return adj_launch(a, a->d_cfgs_own, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
+    EM_TRY(adj_launch(a, a->d_cfgs_own, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
                       d_ckpt, d_gx, d_gy, d_gz, to_stepp(*step),
-                      reinterpret_cast<float4*>(d_grad_charges), d_grad_seeds, st);
+                      reinterpret_cast<float4*>(d_grad_charges), d_grad_seeds, st));
+    EM_HIP(hipEventRecord(a->done, st));
+    a->last = st;
+    return EM_OK;
 }
 
 namespace em {

@@ -196,7 +196,12 @@ int em_batch_destroy(em_batch* plan);
  * recomputes each checkpoint segment, sweeps it backward, and writes
  *   d_grad_charges : p x 4 floats, dL/d{x, y, z, q} of every charge,
  *   d_grad_seeds   : 3 x n floats (SoA), dL/d(seed point)  (may be NULL).
- * Rules: EM_RULE_PLAIN and EM_RULE_ANGLE (EM_RULE_REF -> EM_ERR_UNSUPPORTED). */
+ * Rules: EM_RULE_PLAIN and EM_RULE_ANGLE (EM_RULE_REF -> EM_ERR_UNSUPPORTED).
+ * Work lists and buffers are kept per device between calls: a call with the
+ * same sizes, shape and ckpt_every reuses them asynchronously (waiting on
+ * the device for the previous call when the stream differs); other calls
+ * rebuild them and block the host until the previous call has finished.
+ * Not for graph capture (use the batch form, em_batch_adjoint). */
 int em_adjoint_dev_f32(const float* d_charges, size_t p, const float* d_x,
                        const float* d_y, const float* d_z, size_t n, size_t len,
                        size_t pitch, const em_step* step, const em_launch* launch,

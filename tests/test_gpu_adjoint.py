@@ -91,6 +91,32 @@ def test_checkpoint_interval_does_not_change_result(gpu):
             assert np.array_equal(gc, ref[0]) and np.array_equal(gs, ref[1]), every
 
 
+def test_repeated_calls_reuse_the_work_lists(gpu):
+    """The single-configuration adjoint keeps its work lists between calls:
+    repeated calls, calls alternating between two problems (which rebuild
+    them), a call with new gradients on the same problem, and a call on a
+    second stream all give the bits of a first call."""
+    wa = gen.plain_config(300, 200, 6.0, 90, seed=5)
+    wb = gen.plain_config(120, 96, 4.0, 50, seed=6)
+    rng = np.random.default_rng(3)
+    Ga, Ga2 = rng.normal(size=(3, wa.len, wa.n)), rng.normal(size=(3, wa.len, wa.n))
+    Gb = rng.normal(size=(3, wb.len, wb.n))
+    rule = rules.angle()
+    first_a = forward_and_adjoint(gpu, wa.charges, wa.seeds, wa.len, rule, Ga, 7)
+    first_a2 = forward_and_adjoint(gpu, wa.charges, wa.seeds, wa.len, rule, Ga2, 7)
+    first_b = forward_and_adjoint(gpu, wb.charges, wb.seeds, wb.len, rule, Gb, 7)
+    for _ in range(2):
+        for w, G, want in ((wa, Ga, first_a), (wa, Ga2, first_a2), (wa, Ga, first_a),
+                           (wb, Gb, first_b)):
+            got = forward_and_adjoint(gpu, w.charges, w.seeds, w.len, rule, G, 7)
+            assert all(np.array_equal(x, y) for x, y in zip(got, want))
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        got = forward_and_adjoint(gpu, wa.charges, wa.seeds, wa.len, rule, Ga, 7)
+    torch.cuda.synchronize()
+    assert all(np.array_equal(x, y) for x, y in zip(got, first_a))
+
+
 def test_adjoint_multi_tile_matches_single_tile(gpu):
     w = gen.plain_config(2000, 128, 12.0, 40, seed=8)
     G = np.random.default_rng(2).normal(size=(3, w.len, w.n)).astype(np.float32)
