@@ -136,9 +136,6 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
         out = s;
         return EM_OK;
     }
-    // With line ends a wave keeps evaluating until its last line has ended (a
-    // whole block, when the charges are re-staged tile by tile): few lines per
-    // wave, and one-wave blocks over small tiles when the charges do not fit.
     const bool restaged = (uint64_t)p > tile_budget(dev, 1);
     if (req && req->lanes_per_line) {
         s.S = req->lanes_per_line;
@@ -173,7 +170,9 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
     }
     // 512-thread blocks once the grid holds two of them per CU (four waves per
     // SIMD): 12 % faster than 256 threads on 16,384 x 65,536 and 8,000 x
-    // 32,768, within 2 % on one-tile problems.
+    // 32,768, within 2 % on one-tile problems.  With line ends a whole block
+    // keeps evaluating until its last line has ended when the charges are
+    // re-staged tile by tile (barriers): one-wave blocks over small tiles then.
     if (req && req->block)
         s.block = req->block;
     else if (stop)
