@@ -115,6 +115,54 @@ def test_full_size_lines_64k_properties(gpu):
     assert torch.equal(rows[:, -1, :], a)
 
 
+def test_full_size_lines_64k_invariances(gpu):
+    """At the headline size with the shipped launch shape (4 lanes, tile 4,096,
+    512 threads, unroll 2), all 2,499 steps: doubling every charge doubles
+    every partial sum exactly, so the rows are bitwise unchanged; unroll 8
+    consumes the records in the same order, so its rows are bitwise equal."""
+    w = gen.lines_64k()
+    c = torch.from_numpy(w.charges).cuda()
+    s = torch.from_numpy(w.seeds).cuda()
+    rule = rules.angle(stop=False)
+    shipped = gpu.make_launch(4, 4096, 512, 2)
+    a = gpu.trace(c, s, w.len, rule, shipped)
+    c2 = c.clone()
+    c2[:, 3] *= 2.0
+    b = gpu.trace(c2, s, w.len, rule, shipped)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    del b
+    u8 = gpu.trace(c, s, w.len, rule, gpu.make_launch(4, 4096, 512, 8))
+    torch.cuda.synchronize()
+    assert torch.equal(a, u8)
+
+
+def test_full_size_batch_512(gpu):
+    """The 512 configurations (1,000 rows) in one graph-replayed persistent
+    launch with the library's default shape: finite, every line all 999 steps,
+    and the configurations with the most and the fewest charges and lines
+    bitwise equal to single-configuration traces with the plan's shape."""
+    ws = gen.batch_512()
+    layout, charges, lines = gpu.pack_batch(ws)
+    rule = rules.angle(stop=False)
+    plan = gpu.BatchPlan(layout, rule, None, use_graph=True)
+    info = plan.info()
+    steps = torch.empty(int(layout.n.sum()), dtype=torch.int32, device="cuda")
+    plan.run(charges, lines, steps)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(lines).all())
+    assert bool((steps == 999).all())
+    launch = gpu.make_launch(info["lanes_per_line"], info["tile"], info["block"], info["unroll"])
+    picks = {int(np.argmax(layout.p)), int(np.argmin(layout.p)), int(np.argmax(layout.n)),
+             int(np.argmin(layout.n))}
+    for i in sorted(picks):
+        w = ws[i]
+        single = gpu.trace(torch.from_numpy(w.charges).cuda(), torch.from_numpy(w.seeds).cuda(),
+                           w.len, rule, launch)
+        assert torch.equal(layout.rows_view(lines, i), single), i
+    plan.close()
+
+
 def test_autotuner_picks_a_valid_shape(gpu, tmp_path, monkeypatch):
     """The tuned shape is one of the candidates, is cached, and gives the same
     lines as an explicit launch with that shape."""
