@@ -257,11 +257,13 @@ static HostPathBuffers g_hp[64];
 // Row chunks of a host-array trace: the copy back of chunk k runs while the
 // kernel of chunk k+1 traces.  Possible for the plain and ref rules, whose
 // state is the point alone (a chunk continues from the previous chunk's last
-// row); one chunk per ~96 MB of output, at most 8.
+// row); one chunk per 48 MiB of output, at most 8.  Only the last chunk's
+// copy is exposed: on the reference's default problem (491 MB of rows) eight
+// chunks take 20.1-20.4 ms per call against 21.7 ms with four.
 static uint32_t host_chunks(const StepP& sp, bool ends, size_t len, size_t out_bytes)
 {
     if (ends || (sp.rule != kRulePlain && sp.rule != kRuleRef) || len < 16) return 1;
-    return (uint32_t)std::min<size_t>(8, std::max<size_t>(1, out_bytes / (96ull << 20)));
+    return (uint32_t)std::min<size_t>(8, std::max<size_t>(1, out_bytes / (48ull << 20)));
 }
 
 static int ensure(void** ptr, size_t* have, size_t need)
