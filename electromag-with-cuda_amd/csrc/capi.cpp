@@ -240,6 +240,9 @@ static int check_sizes(size_t p, size_t n, size_t len, const em_step* st)
 
 // ---------------------------------------------------------------------------
 // device buffer cache for the host-array forms (one set per device)
+constexpr size_t kMaxChunks = 8;                // row chunks of a host-array trace
+constexpr size_t kChunkBytes = 48ull << 20;     // output bytes per row chunk
+
 struct HostPathBuffers {
     void* charges = nullptr;
     size_t charges_bytes = 0;
@@ -250,7 +253,7 @@ struct HostPathBuffers {
     hipStream_t stream = nullptr;
     hipStream_t copy = nullptr;  // copies back of finished row chunks, beside the kernels
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t chunk_ev[8] = {};
+    hipEvent_t chunk_ev[kMaxChunks] = {};
 };
 static HostPathBuffers g_hp[64];
 
@@ -259,11 +262,11 @@ static HostPathBuffers g_hp[64];
 // state is the point alone (a chunk continues from the previous chunk's last
 // row); one chunk per 48 MiB of output, at most 8.  Only the last chunk's
 // copy is exposed: on the reference's default problem (491 MB of rows) eight
-// chunks take 20.1-20.4 ms per call against 21.7 ms with four.
+// chunks take 20.1-20.4 ms per call against 21.7 ms with four (sixteen: within 2 %).
 static uint32_t host_chunks(const StepP& sp, bool ends, size_t len, size_t out_bytes)
 {
     if (ends || (sp.rule != kRulePlain && sp.rule != kRuleRef) || len < 16) return 1;
-    return (uint32_t)std::min<size_t>(8, std::max<size_t>(1, out_bytes / (48ull << 20)));
+    return (uint32_t)std::min<size_t>(kMaxChunks, std::max<size_t>(1, out_bytes / kChunkBytes));
 }
 
 static int ensure(void** ptr, size_t* have, size_t need)
