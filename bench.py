@@ -259,11 +259,12 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
             "mean_steps": taken / w.n,
             "launch": {"lanes_per_line": launch_s.lanes_per_line, "tile": launch_s.tile,
                        "block": launch_s.block, "unroll": launch_s.unroll},
-            "lines_past_first_phase": int((st > 192).sum().item()),
+            "lines_over_192_steps": int((st > 192).sum().item()),
             "note": "lines end at sinks; two phases: lane groups take a new line whenever theirs "
-                    "ends, up to 192 steps (launch = this phase's shape), then 256-lane "
-                    "workgroups (two per CU) trace the lines still moving, up to 8 at a time; rows "
-                    "after a line's end are then filled"}
+                    "ends, up to 192 steps (128 for the last lines handed out, one per group; "
+                    "launch = this phase's shape), then 256-lane workgroups (two per CU) trace "
+                    "the lines still moving, up to 8 at a time; rows after a line's end are then "
+                    "filled"}
         del rows
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
     if on("batch_512"):

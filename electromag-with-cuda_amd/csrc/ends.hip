@@ -10,7 +10,11 @@
 //   1. k_trace_ends<S, U>: a persistent grid of S-lane groups.  A group whose
 //      line ends takes the next line from a counter; a line still moving
 //      after kEndsCap steps is set aside (its state goes to a list) and the
-//      group moves on.  Charges go through LDS tiles as in the other kernels.
+//      group moves on.  The last lines handed out (one per lane group) are
+//      the ones the phase drains on once the counter is exhausted: they are
+//      set aside after kEndsDrainCap steps already (the lines that end do so
+//      early: in the problem above 99 % of them within 110 steps).  Charges
+//      go through LDS tiles as in the other kernels.
 //   2. k_trace_tail: two 256-lane workgroups (4 waves, one per SIMD) per CU;
 //      each owns a share of the lines set aside and traces up to 8 of them at
 //      once.  The 256 lanes split every charge sum; each charge record a lane
@@ -35,6 +39,8 @@ namespace em {
 namespace {
 
 constexpr uint32_t kEndsCap = 192;     // steps a line may take in the first phase
+constexpr uint32_t kEndsDrainCap = 128;  // ... for the last lines handed out, one per lane group
+constexpr int kSurvRec = 3;            // float4 records per set-aside line
 constexpr int kTailLanes = 256;        // lanes of a second-phase workgroup
 constexpr int kTailWaves = kTailLanes / 64;
 constexpr int kSlots = 8;              // lines a second-phase workgroup traces at once
@@ -66,17 +72,18 @@ __global__ void k_ends_prepare(EndsCtrl* __restrict__ ctrl, const float4* __rest
 }
 
 // First phase (RESUME = false): lines 0..n-1 from their seeds; a line still
-// moving after `cap` steps is set aside in `surv`, two float4 per line:
-// {x, y, z, h} and {ux, uy, uz, line index as bits}.
+// moving after `cap` steps -- `drain` steps for the last lines handed out, one
+// per lane group of the grid -- is set aside in `surv`, kSurvRec float4 per line:
+// {x, y, z, h}, {ux, uy, uz, line index as bits}, {step as bits, 0, 0, 0}.
 // Continuation (RESUME = true): when more lines were set aside than the second
 // phase holds at once (more than `tail_max`), this kernel takes them instead,
-// from step `start` + 1 to the end, with the same refill; otherwise it exits
-// at once (the second phase has them).
+// each from the step after the one it was set aside at, to the end, with the
+// same refill; otherwise it exits at once (the second phase has them).
 template <int S, int U, bool RESUME>
 __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __restrict__ charges,
                                                     float* __restrict__ X, float* __restrict__ Y,
                                                     float* __restrict__ Z, StepP sp,
-                                                    uint32_t tile_cap, uint32_t cap, uint32_t start,
+                                                    uint32_t tile_cap, uint32_t cap, uint32_t drain,
                                                     uint32_t tail_max, uint32_t* __restrict__ steps,
                                                     EndsCtrl* __restrict__ ctrl,
                                                     float4* __restrict__ surv)
@@ -86,6 +93,8 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
     if (RESUME && nsurv <= tail_max) return;  // uniform: before any barrier
     const int k = threadIdx.x % S;
     const int leader = (int)(threadIdx.x % 64) - k;  // wave lane of the group's lane 0
+    const uint32_t groups = gridDim.x * (blockDim.x / S);
+    const uint32_t late = cfg.n > groups ? cfg.n - groups : 0u;
     const uint32_t p = cfg.p;
     const uint32_t pp = round_up(p, S * kGroup);
     const bool single = pp <= tile_cap;
@@ -109,15 +118,16 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
             if (k == 0) got = atomicAdd(RESUME ? &ctrl->next_resume : &ctrl->next_line, 1u);
             got = __shfl(got, leader, 64);
             if (RESUME && got < nsurv) {
-                const float4 a = surv[2ull * got];
-                const float4 b = surv[2ull * got + 1];
+                const float4 a = surv[kSurvRec * (uint64_t)got];
+                const float4 b = surv[kSurvRec * (uint64_t)got + 1];
+                const float4 c = surv[kSurvRec * (uint64_t)got + 2];
                 line = __float_as_uint(b.w);
                 st.px = a.x; st.py = a.y; st.pz = a.z; st.h = a.w;
                 st.ux = b.x; st.uy = b.y; st.uz = b.z;
                 st.Px = st.Py = st.Pz = 0.f;
                 st.alive = true;
-                st.steps = start;
-                s = start + 1;
+                st.steps = __float_as_uint(c.x);
+                s = st.steps + 1;
                 active = true;
             } else if (!RESUME && got < cfg.n) {
                 line = got;
@@ -154,14 +164,20 @@ __global__ void __launch_bounds__(512) k_trace_ends(Cfg cfg, const float4* __res
                     Z[o] = st.pz;
                 }
             }
+            // the last `groups` lines handed out are the ones the phase
+            // drains on: they are set aside at `drain` steps, the others at
+            // `cap` (a rule on the line index, so the rows do not depend on
+            // which group took a line when)
+            const bool aside = !RESUME && st.alive && s + 1 < cfg.len && s == (line >= late ? drain : cap);
             if (!st.alive || s + 1 >= cfg.len) {
                 if (k == 0) steps[cfg.line_index + line] = st.steps;
                 active = false;
-            } else if (!RESUME && s == cap) {
+            } else if (aside) {
                 if (k == 0) {
-                    const uint32_t i = atomicAdd(&ctrl->nsurv, 1u);
-                    surv[2ull * i] = make_float4(st.px, st.py, st.pz, st.h);
-                    surv[2ull * i + 1] = make_float4(st.ux, st.uy, st.uz, __uint_as_float(line));
+                    const uint64_t i = atomicAdd(&ctrl->nsurv, 1u);
+                    surv[kSurvRec * i] = make_float4(st.px, st.py, st.pz, st.h);
+                    surv[kSurvRec * i + 1] = make_float4(st.ux, st.uy, st.uz, __uint_as_float(line));
+                    surv[kSurvRec * i + 2] = make_float4(__uint_as_float(s), 0.f, 0.f, 0.f);
                 }
                 active = false;
             } else {
@@ -260,7 +276,7 @@ struct Slot {
 // front through LDS (`xfer`), their points published in `pts` and their count
 // returned.
 __device__ __forceinline__ uint32_t refill_slots(Slot& sl, bool done, uint32_t nact,
-                                                 uint32_t& next_k, uint32_t nsurv, uint32_t cap,
+                                                 uint32_t& next_k, uint32_t nsurv,
                                                  const float4* __restrict__ surv,
                                                  float4* __restrict__ xfer, float4* __restrict__ pts)
 {
@@ -276,15 +292,16 @@ __device__ __forceinline__ uint32_t refill_slots(Slot& sl, bool done, uint32_t n
     // list position advances by their count (and stops at the list's end)
     next_k += (uint32_t)__popcll(__ballot(got));
     if (got) {
-        const float4 a = surv[2ull * j];
-        const float4 b = surv[2ull * j + 1];
+        const float4 a = surv[kSurvRec * (uint64_t)j];
+        const float4 b = surv[kSurvRec * (uint64_t)j + 1];
+        const float4 c = surv[kSurvRec * (uint64_t)j + 2];
         sl.st.px = a.x; sl.st.py = a.y; sl.st.pz = a.z; sl.st.h = a.w;
         sl.st.ux = b.x; sl.st.uy = b.y; sl.st.uz = b.z;
         sl.st.Px = sl.st.Py = sl.st.Pz = 0.f;
         sl.st.alive = true;
-        sl.st.steps = cap;
+        sl.st.steps = __float_as_uint(c.x);  // the step it was set aside at
         sl.line = __float_as_uint(b.w);
-        sl.s = cap + 1;
+        sl.s = sl.st.steps + 1;
     }
     const bool keep = holds || got;
     const uint64_t keep_mask = __ballot(keep);
@@ -333,7 +350,7 @@ __device__ __forceinline__ uint32_t refill_slots(Slot& sl, bool done, uint32_t n
 __global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4* __restrict__ padded,
                                                            uint32_t pp, float* __restrict__ X,
                                                            float* __restrict__ Y, float* __restrict__ Z,
-                                                           StepP sp, uint32_t cap, uint32_t tail_max,
+                                                           StepP sp, uint32_t tail_max,
                                                            uint32_t* __restrict__ steps,
                                                            const EndsCtrl* __restrict__ ctrl,
                                                            const float4* __restrict__ surv)
@@ -361,7 +378,7 @@ __global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4
     sl.s = 0;
     uint32_t next_k = 0, nact = 0;
     if (wave == 0) {
-        nact = refill_slots(sl, false, 0u, next_k, nsurv, cap, surv, xfer, pts);
+        nact = refill_slots(sl, false, 0u, next_k, nsurv, surv, xfer, pts);
         if (tid == 0) s_nact = nact;
     }
     for (;;) {
@@ -407,7 +424,7 @@ __global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4
             const bool change = __ballot(done) != 0ull ||
                                 (n_now < (uint32_t)kSlots && blockIdx.x + next_k * gridDim.x < nsurv);
             if (change) {
-                nact = refill_slots(sl, done, n_now, next_k, nsurv, cap, surv, xfer, pts);
+                nact = refill_slots(sl, done, n_now, next_k, nsurv, surv, xfer, pts);
             } else {
                 if ((uint32_t)tid < n_now) pts[tid] = make_float4(sl.st.px, sl.st.py, sl.st.pz, 0.f);
                 nact = n_now;
@@ -440,7 +457,7 @@ __global__ void k_ends_fill(Cfg cfg, float* __restrict__ X, float* __restrict__ 
 
 template <int S, int U, bool RESUME>
 int launch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
-                  float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t start,
+                  float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t drain,
                   uint32_t tail_max, uint32_t* steps, EndsCtrl* ctrl, float4* surv,
                   hipStream_t stream)
 {
@@ -455,20 +472,20 @@ int launch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const 
     const uint64_t want = (cfg.n + groups_per_block - 1) / groups_per_block;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * dev.cus, want);
     hipLaunchKernelGGL((k_trace_ends<S, U, RESUME>), dim3(grid), dim3(sh.block), lds, stream, cfg,
-                       charges, x, y, z, sp, sh.tile, cap, start, tail_max, steps, ctrl, surv);
+                       charges, x, y, z, sp, sh.tile, cap, drain, tail_max, steps, ctrl, surv);
     EM_HIP(hipGetLastError());
     return EM_OK;
 }
 
 template <bool RESUME>
 int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
-                    float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t start,
+                    float* x, float* y, float* z, const StepP& sp, uint32_t cap, uint32_t drain,
                     uint32_t tail_max, uint32_t* steps, EndsCtrl* ctrl, float4* surv,
                     hipStream_t stream)
 {
 #define EM_PHASE1_U(SS, UU)                                                                   \
     case UU:                                                                                  \
-        return launch_phase1<SS, UU, RESUME>(sh, dev, cfg, charges, x, y, z, sp, cap, start,   \
+        return launch_phase1<SS, UU, RESUME>(sh, dev, cfg, charges, x, y, z, sp, cap, drain,   \
                                              tail_max, steps, ctrl, surv, stream)
 #define EM_PHASE1(SS)                                                                         \
     case SS:                                                                                  \
@@ -531,11 +548,12 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
     // padded copy is made.
     const bool tail = cfg.p >= kTailMinCharges && cfg.len > kEndsCap + 1;
     const uint32_t cap = tail ? kEndsCap : cfg.len;
+    const uint32_t drain = tail ? kEndsDrainCap : cfg.len;
     const uint32_t pp = tail ? round_up(cfg.p, kTailLanes) : 0u;
     const size_t off_steps = align256(sizeof(EndsCtrl));
     const size_t steps_bytes = steps ? 0 : align256((size_t)cfg.n * sizeof(uint32_t));
     const size_t off_surv = off_steps + steps_bytes;
-    const size_t off_pad = off_surv + align256((size_t)cfg.n * 2 * sizeof(float4));
+    const size_t off_pad = off_surv + align256((size_t)cfg.n * kSurvRec * sizeof(float4));
     const size_t need = off_pad + (size_t)pp * sizeof(float4);
 
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
@@ -571,8 +589,8 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
     EM_HIP(hipGetLastError());
     // the second phase holds 8 lines in each of its 2-per-CU workgroups at once
     const uint32_t tail_max = 2u * (uint32_t)dev.cus * (uint32_t)kSlots;
-    EM_TRY(dispatch_phase1<false>(sh, dev, cfg, charges, x, y, z, sp, cap, 0u, tail_max, st, ctrl,
-                                  surv, stream));
+    EM_TRY(dispatch_phase1<false>(sh, dev, cfg, charges, x, y, z, sp, cap, drain, tail_max, st,
+                                  ctrl, surv, stream));
     if (tail) {
         // two workgroups per CU: each loads every charge record once per step
         // for all the lines it holds
@@ -582,10 +600,10 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
                                                              kTailReserve));
         if (per_cu != 2) EM_FAIL(EM_ERR_INVALID, "second-phase workgroups per CU: %d, not 2", per_cu);
         hipLaunchKernelGGL(k_trace_tail, dim3(2 * dev.cus), dim3(kTailLanes), kTailReserve, stream,
-                           cfg, padded, pp, x, y, z, sp, kEndsCap, tail_max, st, ctrl, surv);
+                           cfg, padded, pp, x, y, z, sp, tail_max, st, ctrl, surv);
         EM_HIP(hipGetLastError());
         // more set-aside lines than that: the first phase's groups continue them
-        EM_TRY(dispatch_phase1<true>(sh, dev, cfg, charges, x, y, z, sp, cfg.len, kEndsCap,
+        EM_TRY(dispatch_phase1<true>(sh, dev, cfg, charges, x, y, z, sp, cfg.len, cfg.len,
                                      tail_max, st, ctrl, surv, stream));
     }
     const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);

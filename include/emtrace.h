@@ -138,10 +138,11 @@ int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x,
  * em_ckpt_floats().  d_steps (may be NULL): n step counts.
  * The angle rule with stop, no checkpoints and p >= 4096 is traced in two
  * phases (lane groups that take a new line when theirs ends, then workgroups
- * that trace the lines still moving after 192 steps, up to 8 at a time, or the
- * lane groups again when more than that many lines are still moving);
+ * that trace the lines still moving after 192 steps -- 128 for the last lines
+ * handed out, one per lane group -- up to 8 at a time, or the lane groups
+ * again when more than that many lines are still moving);
  * `launch` then sets the first phase's shape.
- * That path keeps per-device work memory (about 32 bytes per line plus a
+ * That path keeps per-device work memory (about 52 bytes per line plus a
  * padded copy of the charges), allocated on first use and grown by doubling;
  * outgrown buffers stay allocated, since a graph captured earlier may use
  * them, until em_shutdown.  Under stream capture it must already be large

@@ -1,6 +1,7 @@
 """The two-phase trace of the angle rule with line ends (stop on, no
 checkpoints, at least TAIL_MIN charges): lane groups that take a new line
-whenever theirs ends, up to a cap of CAP steps; then 256-lane workgroups (two
+whenever theirs ends, up to a cap of CAP steps (DRAIN for the last lines
+handed out, one per lane group); then 256-lane workgroups (two
 per CU) that trace the lines still moving, up to 8 at a time -- or, when more
 lines than that (TAIL_SLOTS on a 256-CU MI355X) reach the cap, the first
 phase's groups continue them; then the rows after each line's end are filled
@@ -10,8 +11,8 @@ path (the tests below cover both sides).
 Parity: against the CPU path (identical step counts, points within 1e-4 on
 well-conditioned lines, path distance to FP64 no worse than the CPU float32
 path's), across the cap; bitwise against the one-kernel path (the
-checkpointed forward, same launch shape) on rows up to the cap, where both
-run the same arithmetic in the same order.  Also: determinism under dynamic
+checkpointed forward, same launch shape) on rows up to the drain cap, where
+both run the same arithmetic in the same order.  Also: determinism under dynamic
 line assignment, shapes at the edges of both phases, more lines set aside than
 the second phase holds at once, pitch > n, graph replays, and reuse of the
 work memory across sizes.
@@ -26,6 +27,7 @@ from test_gpu_trace import dev
 
 pytestmark = pytest.mark.gpu
 CAP = 192  # steps of the first phase (the second phase starts at step CAP + 1)
+DRAIN = 128  # ... for the last lines handed out (every line, when no more lines than groups)
 TAIL_MIN = 4096  # fewer charges: the one-kernel path
 TAIL_SLOTS = 2 * 256 * 8  # lines the second phase holds at once on 256 CUs
 
@@ -122,32 +124,46 @@ def test_across_the_cap_vs_cpu_path(gpu, orc):
         assert (g_rows[:, e + 1:, l] == g_rows[:, e:e + 1, l]).all()
 
 
-@pytest.mark.parametrize("source", ["lines_8k", "lines_64k"])
+@pytest.mark.parametrize("source", ["lines_8k", "lines_64k", "lines_64k_all"])
 def test_matches_one_kernel_path_bitwise(gpu, source):
     """Same launch shape as the one-kernel path (the checkpointed forward):
     with fewer than TAIL_MIN charges (lines_8k, 1,000) both calls take the
     one-kernel path and every row is bit-identical; with more (lines_64k,
-    16,384) rows 0..CAP and every line that ends before the cap are, and later
-    rows agree closely."""
+    16,384; fewer lines than lane groups, so every line has the drain cap)
+    rows 0..DRAIN and every line that ends before it are, and later rows
+    agree closely."""
     if source == "lines_8k":
         w = gen.lines_8k()
         charges, seeds = w.charges, w.seeds[::8]
-    else:
+    elif source == "lines_64k":
         charges, seeds = lines_64k_subset()
-    length = 400
+    else:
+        # all 65,536 lines: more lines than lane groups (at most 256 CUs x 5
+        # blocks of 512 threads x 16 groups = 20,480), so the first 40,960
+        # lines keep the full cap and only the last ones take the drain cap
+        w = gen.lines_64k()
+        charges, seeds = w.charges, w.seeds
+    length = 400 if source != "lines_64k_all" else 260
     launch = gpu.make_launch(32, 512, 512)
     a_rows, a_steps = ends_trace(gpu, charges, seeds, length, launch)
     b_rows, b_steps = one_kernel_trace(gpu, charges, seeds, length, launch)
     if charges.shape[0] < TAIL_MIN:
         assert np.array_equal(a_steps, b_steps) and np.array_equal(a_rows, b_rows)
         return
-    short = b_steps < CAP
+    short = b_steps < DRAIN
     assert np.array_equal(a_steps[short], b_steps[short])
-    assert np.array_equal(a_rows[:, :CAP + 1], b_rows[:, :CAP + 1])
+    assert np.array_equal(a_rows[:, :DRAIN + 1], b_rows[:, :DRAIN + 1])
     assert np.array_equal(a_rows[:, :, short], b_rows[:, :, short])
+    if source == "lines_64k_all":
+        early = np.arange(len(b_steps)) < 40960
+        assert np.array_equal(a_rows[:, :CAP + 1, early], b_rows[:, :CAP + 1, early])
+        ended = early & (b_steps < CAP)
+        assert np.array_equal(a_rows[:, :, ended], b_rows[:, :, ended])
+        assert np.array_equal(a_steps[ended], b_steps[ended])
     long_ = ~short
     d = np.abs(a_rows[:, :, long_].astype(np.float64) - b_rows[:, :, long_]).max()
-    print(f"[ends] vs one-kernel: {long_.sum()} lines past the cap, max |diff| {d:.2e}", flush=True)
+    print(f"[ends] vs one-kernel: {long_.sum()} lines past the drain cap, max |diff| {d:.2e}",
+          flush=True)
     assert long_.sum() >= 16
 
 
@@ -202,7 +218,7 @@ def test_many_lines_past_the_cap(gpu, orc, lines):
     check_lines(orc, g_rows, c_rows, ch, seeds, length, f"{lines} escaping", SHORT_STEPS)
 
 
-@pytest.mark.parametrize("length", [2, CAP, CAP + 1, CAP + 2])
+@pytest.mark.parametrize("length", [2, CAP, CAP + 1, CAP + 2, CAP + 60])
 def test_lengths_at_the_cap(gpu, orc, length):
     ch, seeds = escaping(4500, 130, 5.0 * (4500 / 40.0) ** (1.0 / 3.0), length, seed=7)
     ch[::3, 3] *= -1.0  # some sinks: a mix of ending and escaping lines
