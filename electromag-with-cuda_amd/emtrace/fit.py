@@ -65,6 +65,14 @@ class InverseFit:
             o, n, ln = int(self.layout.line_off[i]), int(self.layout.n[i]), int(self.layout.len[i])
             wts[o:o + n * ln] = 2.0 / (n * ln)
         self.wts = torch.from_numpy(wts).cuda()
+        # the same weight per charge: the adjoint is linear in dL/dx and a
+        # configuration's charges see only its own lines, so the iteration
+        # differentiates sum (x - x_target) and scales each charge's gradient
+        # (one pass over the lines fewer than weighting dL/dx itself)
+        cs = np.concatenate([np.full(int(self.layout.p[i]), 2.0 / (int(self.layout.n[i]) *
+                                                                   int(self.layout.len[i])),
+                                     dtype=np.float32) for i in range(len(self.layout.p))])
+        self.cscale = torch.from_numpy(cs).cuda().unsqueeze(1)
         self.m = torch.zeros_like(self.theta)
         self.v = torch.zeros_like(self.theta)
         self.t = torch.zeros((), dtype=torch.float32, device="cuda")
@@ -94,8 +102,8 @@ class InverseFit:
         torch = self.torch
         self.plan.run_ckpt(self.theta, self.lines, self.ckpt, self.every)
         torch.sub(self.lines, self.target, out=self.G)
-        self.G.mul_(self.wts)
         self.plan.adjoint(self.theta, self.lines, self.ckpt, self.every, self.G, self.grad)
+        self.grad.mul_(self.cscale)
         self.t.add_(1.0)
         self.m.mul_(self.b1).add_(self.grad, alpha=1.0 - self.b1)
         self.v.mul_(self.b2).addcmul_(self.grad, self.grad, value=1.0 - self.b2)

@@ -103,6 +103,26 @@ def test_fit_64_subset_gradients_vs_fp64(gpu, orc):
         assert r < 1e-1
 
 
+def test_scaled_gradient_equals_weighted_dldx(gpu):
+    """The iteration differentiates sum (x - x_target) and scales each charge's
+    gradient by its configuration's weight 2 / (n len); the adjoint of the
+    weighted dL/dx gives the same gradient up to float rounding."""
+    ws = gen.fit_64()[:6]
+    truth = [gen.perturbed(w.charges, 0.1, seed=i) for i, w in enumerate(ws)]
+    fit = InverseFit(ws, truth, rules.angle(), ckpt_every=50)
+    fit.plan.run_ckpt(fit.theta, fit.lines, fit.ckpt, fit.every)
+    torch.sub(fit.lines, fit.target, out=fit.G)
+    fit.plan.adjoint(fit.theta, fit.lines, fit.ckpt, fit.every, fit.G, fit.grad)
+    scaled = (fit.grad * fit.cscale).cpu().numpy().astype(np.float64)
+    fit.G.mul_(fit.wts)
+    fit.plan.adjoint(fit.theta, fit.lines, fit.ckpt, fit.every, fit.G, fit.grad)
+    torch.cuda.synchronize()
+    weighted = fit.grad.cpu().numpy().astype(np.float64)
+    rel = np.linalg.norm(scaled - weighted) / np.linalg.norm(weighted)
+    print(f"[fit] scaled vs weighted gradient: rel {rel:.2e}", flush=True)
+    assert rel < 1e-5
+
+
 def test_short_fit_lowers_the_loss(gpu):
     ws, truth = _problem(8, (128, 512), (256, 512), 200, 0.1, 5)
     fit = InverseFit(ws, truth, rules.angle(), ckpt_every=50)
