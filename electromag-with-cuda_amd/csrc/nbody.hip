@@ -19,7 +19,8 @@
 // (electro::dynamicPointCharge<float>: pointCharge, velocity, mass); static
 // charges are {x, y, z, q}; accelerations are n x 3 floats (Array<Vector3>).
 // S lanes per particle (S = 1..8: until the grid holds four waves per SIMD;
-// 65,536 particles among 16,384 charges: 3.15-3.23 ms at S = 4 against
+// 65,536 particles among 16,384 charges: 3.03-3.05 ms at S = 4 with 2-record
+// pipelined groups (4 records: 3.14-3.20 ms, 8: 3.41-3.46 ms), against
 // 3.81-3.89 ms at S = 1, where one wave per SIMD leaves every latency
 // exposed).  Static charges go through the tracer's pipelined tile walk
 // (FieldBody).  Dynamic sources are staged as two float4 records,
@@ -228,7 +229,7 @@ int launch_verlet(const float* parts, uint32_t n, const float* charges, uint32_t
     switch (S) {
         case 1: return launch_verlet_s<1, 8>(parts, n, charges, p, accel, vp, out, tile, stream);
         case 2: return launch_verlet_s<2, 4>(parts, n, charges, p, accel, vp, out, tile, stream);
-        case 4: return launch_verlet_s<4, 4>(parts, n, charges, p, accel, vp, out, tile, stream);
+        case 4: return launch_verlet_s<4, 2>(parts, n, charges, p, accel, vp, out, tile, stream);
         default: return launch_verlet_s<8, 4>(parts, n, charges, p, accel, vp, out, tile, stream);
     }
 }
