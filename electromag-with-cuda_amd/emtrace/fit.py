@@ -33,11 +33,13 @@ class InverseFit:
         if launch is None:
             # With line ends most lines are short and the few full-length lines
             # set the time of each pass: give every line as many lanes as the
-            # largest configuration keeps busy (>= 32 charges per lane, <= 16).
+            # largest configuration keeps busy (>= 32 charges per lane, <= 16),
+            # in 128-thread work items (fit_64: 21.6-21.7 ms per iteration,
+            # against 22.2-22.4 with 256 threads, 24.8 with 512, 25.6 with 64)
             S = 1
             while S < 16 and int(self.layout.p.max()) // (2 * S) >= 32:
                 S *= 2
-            launch = api.make_launch(S, 0, 256)
+            launch = api.make_launch(S, 0, 128)
         self.theta = charges.clone()                       # parameters (sum p, 4)
         self.theta0 = charges.clone()
         self.true = torch.from_numpy(np.concatenate(target_charges)).cuda()
