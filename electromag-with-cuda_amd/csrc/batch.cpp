@@ -183,8 +183,12 @@ int em_batch_create(const em_batch_cfg* cfgs, size_t count, const em_step* step,
     // 256-thread blocks: with small tiles four workgroups share a CU, so while
     // one re-stages a tile or finishes a short item the others keep issuing
     // (512 configurations, up to 8,192 charges: 432 ms against 452-467 ms with
-    // 512 threads and tiles for two workgroups per CU)
-    if (!req.block) req.block = 256;
+    // 512 threads and tiles for two workgroups per CU).  With line ends a
+    // block runs until its longest line has ended: 128 threads (the same 512
+    // configurations: 103 ms against 112 ms with 256 threads; 64
+    // configurations of <= 1,024 charges, checkpointed forward + adjoint in
+    // the inverse fit: 21.6 against 22.2 ms per iteration)
+    if (!req.block) req.block = step->stop ? 128 : 256;
     Shape sh;
     int rc = choose_shape(dev, max_p, total_lines, &req, sh);
     if (rc) {
