@@ -322,3 +322,160 @@ def test_verlet64_newton_third_law_and_box(orc):
     assert np.allclose(out[0, :3], [2 * 1 - 1.05, 0, 0]) and np.allclose(out[0, 4:7], [-10, 0, 0])
     a, none = orc.verlet64(Q, np.zeros((0, 4)), np.zeros((1, 3)), [-1] * 3, [1] * 3, 0.01, False)
     assert none is None
+
+
+# ---------------------------------------------------------------------------
+# pins for the parity metrics (o64_min_dist_f32, o64_path_dist_f32) and for
+# o64_trace's plain rule.  Each is checked against a formulation that does not
+# share its code: numpy brute force, cross-product segment distance, geometry
+# with a known answer.
+
+def _seg_dist_cross(p, a, b):
+    """Distance from p to segment ab: the end points when the foot of the
+    perpendicular falls outside, else |(p - a) x (b - a)| / |b - a|."""
+    ab = b - a
+    L = np.linalg.norm(ab)
+    da, db = np.linalg.norm(p - a), np.linalg.norm(p - b)
+    if L == 0.0:
+        return da
+    if np.dot(p - a, ab) <= 0.0:
+        return da
+    if np.dot(p - b, -ab) <= 0.0:
+        return db
+    return np.linalg.norm(np.cross(p - a, ab)) / L
+
+
+def test_min_dist_matches_brute_force(orc):
+    """o64_min_dist_f32 (the "> 0.05 from any charge" filter) vs numpy."""
+    rng = np.random.default_rng(12)
+    ch = np.concatenate([rng.uniform(-3, 3, (23, 3)), rng.uniform(-1, 1, (23, 1))],
+                        axis=1).astype(np.float32)
+    rows = rng.uniform(-4, 4, (3, 11, 7)).astype(np.float32)
+    got = orc.min_dist(ch, rows)
+    pts = rows.transpose(1, 2, 0).astype(np.float64)            # (len, n, 3)
+    want = np.sqrt(((pts[:, :, None, :] - ch[None, None, :, :3].astype(np.float64)) ** 2)
+                   .sum(-1)).min(-1)
+    assert got.shape == (11, 7)
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+    # known distances: points 0.03 / 0.05 / 0.07 along the axes of an isolated
+    # charge (the others 100 away)
+    far = np.array([[0.5, -0.25, 1.0, 1.0], [100.0, 0, 0, 1], [0, 100.0, 0, -1]], np.float32)
+    r = np.array([0.03, 0.05, 0.07])
+    pts = far[0, :3] + r[:, None] * np.eye(3)                      # (3 rows, xyz)
+    rows = pts.T[:, :, None].astype(np.float32)                   # (3, len=3, n=1)
+    d = orc.min_dist(far, rows)[:, 0]
+    np.testing.assert_allclose(d, r, rtol=2e-5)
+    assert d[0] <= 0.05 + 1e-6 and d[2] > 0.05
+
+
+def test_path_dist_matches_brute_force(orc):
+    """With a window covering the whole line, o64_path_dist_f32 is the
+    distance to the nearest segment of the reference polyline before first[l];
+    0 at and after first[l]."""
+    rng = np.random.default_rng(13)
+    length, n = 12, 5
+    ref = np.cumsum(rng.normal(size=(3, length, n)), axis=1).astype(np.float32)
+    test = (ref + rng.normal(scale=0.4, size=ref.shape)).astype(np.float32)
+    first = np.array([12, 7, 2, 12, 5])
+    got = orc.path_dist(test, ref, first, window=length)
+    for l in range(n):
+        for j in range(length):
+            if j >= first[l]:
+                assert got[j, l] == 0.0
+                continue
+            p = test[:, j, l].astype(np.float64)
+            want = min(_seg_dist_cross(p, ref[:, k, l].astype(np.float64),
+                                       ref[:, k + 1, l].astype(np.float64))
+                       for k in range(first[l] - 1))
+            assert abs(got[j, l] - want) <= 1e-6 * max(1.0, want), (j, l, got[j, l], want)
+
+
+def test_path_dist_geometry(orc):
+    """Known answers: points on the polyline (0), at a perpendicular offset from
+    a straight polyline (the offset), beyond its end (distance to the end
+    point), and the window bounding the search (a circle traversed once: the
+    diametrically opposite point is found only when the window reaches it)."""
+    length = 40
+    s = np.arange(length) * 0.1
+    ref = np.zeros((3, length, 1), np.float32)
+    ref[0, :, 0] = s                                          # straight along x
+    first = np.array([length])
+    # midpoints of the segments: on the path
+    mid = ref.copy()
+    mid[0, :-1, 0] = s[:-1] + 0.05
+    assert orc.path_dist(mid, ref, first, 4)[:-1].max() < 1e-7
+    # perpendicular offset 0.02 in y and z: distance 0.02 * sqrt(2)
+    off = ref.copy()
+    off[1] += 0.02
+    off[2] -= 0.02
+    np.testing.assert_allclose(orc.path_dist(off, ref, first, 4)[:, 0], 0.02 * np.sqrt(2), rtol=1e-5)
+    # shifted along the line by 0.37 (3.7 steps): inside the window the distance
+    # is 0; the last points lie beyond the end, at (s + 0.37) - s_end
+    sh = ref.copy()
+    sh[0] += 0.37
+    d = orc.path_dist(sh, ref, first, 8)[:, 0]
+    beyond = s + 0.37 > s[-1]
+    assert d[~beyond].max() < 1e-6
+    np.testing.assert_allclose(d[beyond], s[beyond] + 0.37 - s[-1], rtol=1e-5)
+    # the same shift with a window of 2 segments: the foot lies 3.7 segments
+    # ahead, outside [j-2, j+2], so the distance is to point j+2: 0.17
+    d2 = orc.path_dist(sh, ref, first, 2)[:, 0]
+    np.testing.assert_allclose(d2[:30], 0.17, rtol=1e-5)
+    # circle of radius 1 traversed once; the test point j sits where the
+    # reference is half a turn later
+    ang = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    circ = np.zeros((3, 64, 1), np.float32)
+    circ[0, :, 0], circ[1, :, 0] = np.cos(ang), np.sin(ang)
+    opp = np.roll(circ, -32, axis=1)
+    f = np.array([64])
+    wide = orc.path_dist(opp, circ, f, 64)[:, 0]
+    narrow = orc.path_dist(opp, circ, f, 4)[:, 0]
+    assert wide.max() < 1e-6
+    assert narrow.min() > 1.5      # the 4-segment window sees only points near angle j
+
+
+def test_path_dist_pointwise_fallback(orc):
+    """Where the window holds no segment (window 0, or a line with one point
+    before first[l]) the distance is pointwise, to the point of the same index."""
+    rng = np.random.default_rng(14)
+    ref = rng.normal(size=(3, 9, 3)).astype(np.float32)
+    test = rng.normal(size=(3, 9, 3)).astype(np.float32)
+    pw = np.linalg.norm(test.astype(np.float64) - ref.astype(np.float64), axis=0)
+    d0 = orc.path_dist(test, ref, np.array([9, 9, 9]), 0)
+    np.testing.assert_allclose(d0, pw, rtol=1e-6)
+    d1 = orc.path_dist(test, ref, np.array([1, 9, 1]), 3)
+    np.testing.assert_allclose(d1[0, [0, 2]], pw[0, [0, 2]], rtol=1e-6)
+    assert (d1[1:, [0, 2]] == 0).all()
+
+
+def test_trace64_plain_closed_form(orc):
+    """o64_trace, plain rule: one charge gives radial lines with
+    x_s = c + (r0 + s/res) u (CPU_Implement.cpp:89-92, Vector.h:158-164),
+    outward for q > 0 and inward for q < 0."""
+    c = np.array([[1.25, -0.5, 2.0, 0.8]], np.float64)
+    d = gen.fibonacci_dirs(48)
+    for res in (10.0, 4.0):
+        rows = orc.trace64(c, c[0, :3] + 0.1 * d, 60, rules.plain(res))
+        s = np.arange(60)[:, None, None]
+        np.testing.assert_allclose(rows, c[0, :3] + (0.1 + s / res) * d[None], atol=1e-12)
+    cn = c.copy()
+    cn[0, 3] = -0.8
+    rows, steps = orc.trace64(cn, c[0, :3] + 5.0 * d, 30, rules.plain(10.0), with_steps=True)
+    s = np.arange(30)[:, None, None]
+    np.testing.assert_allclose(rows, c[0, :3] + (5.0 - s * 0.1) * d[None], atol=1e-12)
+    assert (steps == 29).all()
+
+
+def test_trace64_plain_matches_reference_double_rows(orc):
+    """o64_trace, plain rule, vs the reference's own CalcField_CPU<double>
+    rows (golden vectors written by the reference compiled from source), before
+    each line's first approach within 0.05 of a charge."""
+    g = np.load(os.path.join(GOLDEN, "ref_f64_plain_p64_n64_len200.npz"))
+    ref = g["rows"]
+    rows = rows64_to_soa(orc.trace64(g["charges"], g["seeds"], ref.shape[1],
+                                     rules.plain(float(g["resolution"]))))
+    first, _ = prefix_lengths(ref.astype(np.float32), g["charges"].astype(np.float32))
+    err = np.abs(rows - ref).max(axis=0)
+    perr = np.array([err[: first[l], l].max() if first[l] else 0.0 for l in range(ref.shape[2])])
+    assert (first > 10).mean() > 0.5
+    assert perr.max() < 1e-8, perr.max()
