@@ -432,7 +432,10 @@ int em_init(int device)
     if (device < 0 || device >= count) EM_FAIL(EM_ERR_INVALID, "device %d of %d", device, count);
     EM_HIP(hipSetDevice(device));
     DeviceInfo d;
-    return current_device_info(d);
+    EM_TRY(current_device_info(d));
+    // the device address of em_trace_dev_f32_noalloc's fixed work memory
+    // (looked up here once per device, else on that entry point's first call)
+    return noalloc_scratch(device, nullptr);
 }
 
 int em_shutdown(void)
@@ -486,12 +489,11 @@ size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every)
     return 8ull * n * ((len - 1) / ckpt_every + 1);
 }
 
-int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, float* d_z,
+static int trace_dev(const float* d_charges, size_t p, float* d_x, float* d_y, float* d_z,
                      size_t n, size_t len, size_t pitch, const em_step* step,
                      const em_launch* launch, float* d_ckpt, uint32_t ckpt_every,
-                     uint32_t* d_steps, void* stream)
+                     uint32_t* d_steps, void* stream, bool noalloc)
 {
-    clear_error();
     EM_TRY(check_sizes(p, n, len, step));
     if ((p && !d_charges) || !d_x || !d_y || !d_z) EM_FAIL(EM_ERR_INVALID, "NULL device pointer");
     if (pitch < n) EM_FAIL(EM_ERR_INVALID, "pitch %zu < n %zu", pitch, n);
@@ -509,12 +511,36 @@ int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, f
     cfg.p = (uint32_t)p;
     cfg.n = (uint32_t)n;
     cfg.len = (uint32_t)len;
+    if (ends && noalloc)
+        return launch_trace_ends_noalloc(sh, dev, cfg, reinterpret_cast<const float4*>(d_charges),
+                                         d_x, d_y, d_z, to_stepp(*step), d_steps,
+                                         static_cast<hipStream_t>(stream));
     if (ends)
         return launch_trace_ends(sh, dev, cfg, reinterpret_cast<const float4*>(d_charges), d_x, d_y,
                                  d_z, to_stepp(*step), d_steps, static_cast<hipStream_t>(stream));
     return launch_trace_direct(sh, cfg, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
                                to_stepp(*step), d_ckpt, ckpt_every ? ckpt_every : 1, d_steps,
                                static_cast<hipStream_t>(stream));
+}
+
+int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y, float* d_z,
+                     size_t n, size_t len, size_t pitch, const em_step* step,
+                     const em_launch* launch, float* d_ckpt, uint32_t ckpt_every,
+                     uint32_t* d_steps, void* stream)
+{
+    clear_error();
+    return trace_dev(d_charges, p, d_x, d_y, d_z, n, len, pitch, step, launch, d_ckpt, ckpt_every,
+                     d_steps, stream, false);
+}
+
+int em_trace_dev_f32_noalloc(const float* d_charges, size_t p, float* d_x, float* d_y,
+                             float* d_z, size_t n, size_t len, size_t pitch,
+                             const em_step* step, const em_launch* launch, float* d_ckpt,
+                             uint32_t ckpt_every, uint32_t* d_steps, void* stream)
+{
+    clear_error();
+    return trace_dev(d_charges, p, d_x, d_y, d_z, n, len, pitch, step, launch, d_ckpt, ckpt_every,
+                     d_steps, stream, true);
 }
 
 int em_field_dev_f32(const float* d_charges, size_t p, const float* d_px, const float* d_py,

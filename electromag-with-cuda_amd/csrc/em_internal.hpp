@@ -119,6 +119,16 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
                       float* x, float* y, float* z, const StepP& sp, uint32_t* steps,
                       hipStream_t stream);
 int ends_release_all();
+// The same trace over a fixed device array of kNoAllocScratchBytes per device
+// (em_trace_dev_f32_noalloc): no allocation; problems whose work memory
+// exceeds the array fail with EM_ERR_INVALID before any launch.
+constexpr size_t kNoAllocScratchBytes = EM_NOALLOC_SCRATCH_BYTES;
+int launch_trace_ends_noalloc(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg,
+                              const float4* charges, float* x, float* y, float* z, const StepP& sp,
+                              uint32_t* steps, hipStream_t stream);
+// The device address of that array on `device` (which must be current),
+// looked up once per device with hipGetSymbolAddress.
+int noalloc_scratch(int device, char** out);
 int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,
                  const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,
                  float* ez, hipStream_t stream);

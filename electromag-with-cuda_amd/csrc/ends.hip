@@ -529,11 +529,34 @@ EndsScratch g_ends[64];
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-}  // namespace
+// Where the pieces of the work memory lie for one problem, and how much it
+// needs.
+struct EndsLayout {
+    bool tail;        // the second phase runs
+    uint32_t cap, drain, pp;
+    size_t off_steps, off_surv, off_pad, need;
+};
 
-int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
-                      float* x, float* y, float* z, const StepP& sp, uint32_t* steps,
-                      hipStream_t stream)
+EndsLayout ends_layout(const Cfg& cfg, bool own_steps)
+{
+    EndsLayout L;
+    // The second phase pays off when a line's step is a long sum.  Below
+    // kTailMinCharges charges the first phase's wide groups take every line
+    // to its end (a step is then at most 128 records per lane), and no
+    // padded copy is made.
+    L.tail = cfg.p >= kTailMinCharges && cfg.len > kEndsCap + 1;
+    L.cap = L.tail ? kEndsCap : cfg.len;
+    L.drain = L.tail ? kEndsDrainCap : cfg.len;
+    L.pp = L.tail ? round_up(cfg.p, kTailLanes) : 0u;
+    L.off_steps = align256(sizeof(EndsCtrl));
+    const size_t steps_bytes = own_steps ? align256((size_t)cfg.n * sizeof(uint32_t)) : 0;
+    L.off_surv = L.off_steps + steps_bytes;
+    L.off_pad = L.off_surv + align256((size_t)cfg.n * kSurvRec * sizeof(float4));
+    L.need = L.off_pad + (size_t)L.pp * sizeof(float4);
+    return L;
+}
+
+int ends_check(const DeviceInfo& dev, const Cfg& cfg)
 {
     if (dev.device < 0 || dev.device >= 64) EM_FAIL(EM_ERR_INVALID, "device %d", dev.device);
     // the line counter passes n by up to one fetch per lane group, and the
@@ -542,19 +565,103 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
     // the second phase addresses the padded charges through a buffer
     // descriptor whose size field counts bytes in 32 bits
     if (cfg.p > (1u << 28) - kTailLanes) EM_FAIL(EM_ERR_INVALID, "line ends: p %u above 2^28 - 256", cfg.p);
-    // The second phase pays off when a line's step is a long sum.  Below
-    // kTailMinCharges charges the first phase's wide groups take every line
-    // to its end (a step is then at most 128 records per lane), and no
-    // padded copy is made.
-    const bool tail = cfg.p >= kTailMinCharges && cfg.len > kEndsCap + 1;
-    const uint32_t cap = tail ? kEndsCap : cfg.len;
-    const uint32_t drain = tail ? kEndsDrainCap : cfg.len;
-    const uint32_t pp = tail ? round_up(cfg.p, kTailLanes) : 0u;
-    const size_t off_steps = align256(sizeof(EndsCtrl));
-    const size_t steps_bytes = steps ? 0 : align256((size_t)cfg.n * sizeof(uint32_t));
-    const size_t off_surv = off_steps + steps_bytes;
-    const size_t off_pad = off_surv + align256((size_t)cfg.n * kSurvRec * sizeof(float4));
-    const size_t need = off_pad + (size_t)pp * sizeof(float4);
+    if ((cfg.len + kFillRows - 1) / kFillRows > 65535u)
+        EM_FAIL(EM_ERR_INVALID, "len %u exceeds the fill grid", cfg.len);
+    return EM_OK;
+}
+
+// The launches of one two-phase trace over work memory `buf` (laid out by
+// `L`).  Allocates nothing; the counters are zeroed by a kernel, so the
+// sequence may be captured into a graph.
+int ends_launch(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
+                float* x, float* y, float* z, const StepP& sp, uint32_t* steps, hipStream_t stream,
+                char* buf, const EndsLayout& L)
+{
+    EndsCtrl* ctrl = reinterpret_cast<EndsCtrl*>(buf);
+    uint32_t* st = steps ? steps : reinterpret_cast<uint32_t*>(buf + L.off_steps);
+    float4* surv = reinterpret_cast<float4*>(buf + L.off_surv);
+    float4* padded = reinterpret_cast<float4*>(buf + L.off_pad);
+
+    hipLaunchKernelGGL(k_ends_prepare, dim3(std::max<uint32_t>(1u, (L.pp + 255) / 256)), dim3(256), 0,
+                       stream, ctrl, charges, cfg.p, padded, L.pp);
+    EM_HIP(hipGetLastError());
+    // the second phase holds 8 lines in each of its 2-per-CU workgroups at once
+    const uint32_t tail_max = 2u * (uint32_t)dev.cus * (uint32_t)kSlots;
+    EM_TRY(dispatch_phase1<false>(sh, dev, cfg, charges, x, y, z, sp, L.cap, L.drain, tail_max, st,
+                                  ctrl, surv, stream));
+    if (L.tail) {
+        // two workgroups per CU: each loads every charge record once per step
+        // for all the lines it holds
+        EM_TRY(allow_lds(k_trace_tail, kTailReserve));
+        int per_cu = 0;
+        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_tail, kTailLanes,
+                                                             kTailReserve));
+        if (per_cu != 2) EM_FAIL(EM_ERR_INVALID, "second-phase workgroups per CU: %d, not 2", per_cu);
+        hipLaunchKernelGGL(k_trace_tail, dim3(2 * dev.cus), dim3(kTailLanes), kTailReserve, stream,
+                           cfg, padded, L.pp, x, y, z, sp, tail_max, st, ctrl, surv);
+        EM_HIP(hipGetLastError());
+        // more set-aside lines than that: the first phase's groups continue them
+        EM_TRY(dispatch_phase1<true>(sh, dev, cfg, charges, x, y, z, sp, cfg.len, cfg.len,
+                                     tail_max, st, ctrl, surv, stream));
+    }
+    const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
+    hipLaunchKernelGGL(k_ends_fill, fill_grid, dim3(256), 0, stream, cfg, x, y, z, st);
+    EM_HIP(hipGetLastError());
+    return EM_OK;
+}
+
+// Work memory of em_trace_dev_f32_noalloc: one fixed array in the device code,
+// so a call allocates nothing.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): each device's
+// address comes from hipGetSymbolAddress, once per device, and is kept in
+// g_scratch_ptr.  Every call on a device shares its copy; it keeps the last
+// call's contents (each call zeroes its counters with a kernel first).
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[64] = {};
+
+}  // namespace
+
+int noalloc_scratch(int device, char** out)
+{
+    if (device < 0 || device >= 64) EM_FAIL(EM_ERR_INVALID, "device %d", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        EM_HIP(hipGetDevice(&cur));
+        if (cur != device) EM_FAIL(EM_ERR_INVALID, "scratch of device %d asked with device %d current", device, cur);
+        void* ptr = nullptr;
+        EM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) EM_FAIL(EM_ERR_HIP, "hipGetSymbolAddress(scratch) gave NULL on device %d", device);
+        g_scratch_ptr[device] = static_cast<char*>(ptr);
+    }
+    if (out) *out = g_scratch_ptr[device];
+    return EM_OK;
+}
+
+int launch_trace_ends_noalloc(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg,
+                              const float4* charges, float* x, float* y, float* z, const StepP& sp,
+                              uint32_t* steps, hipStream_t stream)
+{
+    EM_TRY(ends_check(dev, cfg));
+    const EndsLayout L = ends_layout(cfg, steps == nullptr);
+    if (L.need > kNoAllocScratchBytes)
+        EM_FAIL(EM_ERR_INVALID,
+                "line ends without allocation: %u lines and %u charges need %zu bytes of work "
+                "memory, above the fixed %zu", cfg.n, cfg.p, L.need, kNoAllocScratchBytes);
+    char* buf = nullptr;
+    EM_TRY(noalloc_scratch(dev.device, &buf));
+    return ends_launch(sh, dev, cfg, charges, x, y, z, sp, steps, stream, buf, L);
+}
+
+int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
+                      float* x, float* y, float* z, const StepP& sp, uint32_t* steps,
+                      hipStream_t stream)
+{
+    EM_TRY(ends_check(dev, cfg));
+    const EndsLayout L = ends_layout(cfg, steps == nullptr);
+    const size_t need = L.need;
 
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
     EM_HIP(hipStreamIsCapturing(stream, &capture));
@@ -578,38 +685,7 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
         w.bytes = size;
     }
     if (w.recorded && !capturing) EM_HIP(hipStreamWaitEvent(stream, w.done, 0));
-
-    EndsCtrl* ctrl = reinterpret_cast<EndsCtrl*>(w.buf);
-    uint32_t* st = steps ? steps : reinterpret_cast<uint32_t*>(w.buf + off_steps);
-    float4* surv = reinterpret_cast<float4*>(w.buf + off_surv);
-    float4* padded = reinterpret_cast<float4*>(w.buf + off_pad);
-
-    hipLaunchKernelGGL(k_ends_prepare, dim3(std::max<uint32_t>(1u, (pp + 255) / 256)), dim3(256), 0,
-                       stream, ctrl, charges, cfg.p, padded, pp);
-    EM_HIP(hipGetLastError());
-    // the second phase holds 8 lines in each of its 2-per-CU workgroups at once
-    const uint32_t tail_max = 2u * (uint32_t)dev.cus * (uint32_t)kSlots;
-    EM_TRY(dispatch_phase1<false>(sh, dev, cfg, charges, x, y, z, sp, cap, drain, tail_max, st,
-                                  ctrl, surv, stream));
-    if (tail) {
-        // two workgroups per CU: each loads every charge record once per step
-        // for all the lines it holds
-        EM_TRY(allow_lds(k_trace_tail, kTailReserve));
-        int per_cu = 0;
-        EM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_tail, kTailLanes,
-                                                             kTailReserve));
-        if (per_cu != 2) EM_FAIL(EM_ERR_INVALID, "second-phase workgroups per CU: %d, not 2", per_cu);
-        hipLaunchKernelGGL(k_trace_tail, dim3(2 * dev.cus), dim3(kTailLanes), kTailReserve, stream,
-                           cfg, padded, pp, x, y, z, sp, tail_max, st, ctrl, surv);
-        EM_HIP(hipGetLastError());
-        // more set-aside lines than that: the first phase's groups continue them
-        EM_TRY(dispatch_phase1<true>(sh, dev, cfg, charges, x, y, z, sp, cfg.len, cfg.len,
-                                     tail_max, st, ctrl, surv, stream));
-    }
-    const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
-    if (fill_grid.y > 65535u) EM_FAIL(EM_ERR_INVALID, "len %u exceeds the fill grid", cfg.len);
-    hipLaunchKernelGGL(k_ends_fill, fill_grid, dim3(256), 0, stream, cfg, x, y, z, st);
-    EM_HIP(hipGetLastError());
+    EM_TRY(ends_launch(sh, dev, cfg, charges, x, y, z, sp, steps, stream, w.buf, L));
     if (!capturing) {
         EM_HIP(hipEventRecord(w.done, stream));
         w.recorded = true;

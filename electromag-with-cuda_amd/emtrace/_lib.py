@@ -83,6 +83,8 @@ SIGNATURES = {
                               ctypes.POINTER(em_launch), ctypes.POINTER(em_perf), P]),
     "em_trace_dev_f32": (I, [P, SZ, P, P, P, SZ, SZ, SZ, ctypes.POINTER(em_step),
                              ctypes.POINTER(em_launch), P, U32, P, P]),
+    "em_trace_dev_f32_noalloc": (I, [P, SZ, P, P, P, SZ, SZ, SZ, ctypes.POINTER(em_step),
+                                     ctypes.POINTER(em_launch), P, U32, P, P]),
     "em_ckpt_floats": (SZ, [SZ, SZ, U32]),
     "em_field_dev_f32": (I, [P, SZ, P, P, P, SZ, F, P, P, P, ctypes.POINTER(em_launch), P]),
     "em_batch_create": (I, [ctypes.POINTER(em_batch_cfg), SZ, ctypes.POINTER(em_step),

@@ -172,9 +172,12 @@ def ckpt_size(n: int, length: int, every: int) -> int:
 
 
 def trace_into(charges, rows, n: int, rule: StepRule, launch=None, ckpt=None, ckpt_every: int = 0,
-               steps=None):
+               steps=None, noalloc: bool = False):
     """Trace in place on device rows (3, len, pitch); row 0 must hold the seeds.
-    ``steps`` (optional int32 device tensor of n) receives each line's step count."""
+    ``steps`` (optional int32 device tensor of n) receives each line's step count.
+    ``noalloc`` calls ``em_trace_dev_f32_noalloc``: the same rows, with the
+    line-ends work memory taken from the library's fixed per-device array
+    (no device allocation; larger problems raise EmError)."""
     torch = _torch()
     _dev_check(charges, "charges", torch.float32)
     _dev_check(rows, "rows", torch.float32)
@@ -196,9 +199,11 @@ def trace_into(charges, rows, n: int, rule: StepRule, launch=None, ckpt=None, ck
         if steps.numel() < n:
             raise ValueError("steps buffer too small")
         sp = steps.data_ptr()
-    call("em_trace_dev_f32", charges.data_ptr(), charges.shape[0], rows[0].data_ptr(),
-         rows[1].data_ptr(), rows[2].data_ptr(), n, length, pitch, ctypes.byref(st),
-         _launch_arg(launch), cp, ckpt_every, sp, _stream_handle())
+    if not 0 < n <= pitch:
+        raise ValueError("need 0 < n <= pitch")
+    call("em_trace_dev_f32_noalloc" if noalloc else "em_trace_dev_f32", charges.data_ptr(),
+         charges.shape[0], rows[0].data_ptr(), rows[1].data_ptr(), rows[2].data_ptr(), n, length,
+         pitch, ctypes.byref(st), _launch_arg(launch), cp, ckpt_every, sp, _stream_handle())
     return rows
 
 

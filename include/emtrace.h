@@ -157,6 +157,31 @@ int em_trace_dev_f32(const float* d_charges, size_t p, float* d_x, float* d_y,
 
 size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every);
 
+/* ---- allocation-free form of em_trace_dev_f32 -------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under stream/graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, same rows and step counts as
+ * em_trace_dev_f32 (the same kernels), but the work memory of the two-phase
+ * line-ends trace is a fixed array in the library's device code,
+ * EM_NOALLOC_SCRATCH_BYTES per device, held from the first call on a device
+ * (or em_init) until the process ends.  The call allocates nothing, creates
+ * no stream or event, and never waits on the host.
+ * Work memory needed by that path (angle rule with stop, no checkpoints,
+ * p >= 4096): 256 + a(48 n) + 16 round_up(p, 256) bytes, plus a(4 n) when
+ * d_steps is NULL, where a() rounds up to 256; more than
+ * EM_NOALLOC_SCRATCH_BYTES -> EM_ERR_INVALID before any launch (with steps
+ * given: about 170,000 lines at 16,384 charges).  The other paths use no work
+ * memory.
+ * Every call on one device uses that device's one array: calls that overlap
+ * on a device (on different streams, or a graph replay beside another call)
+ * race with each other; calls on one stream are ordered. */
+#define EM_NOALLOC_SCRATCH_BYTES (8u << 20)
+int em_trace_dev_f32_noalloc(const float* d_charges, size_t p, float* d_x, float* d_y,
+                             float* d_z, size_t n, size_t len, size_t pitch,
+                             const em_step* step, const em_launch* launch,
+                             float* d_ckpt, uint32_t ckpt_every, uint32_t* d_steps,
+                             void* stream);
+
 /* E(x) = scale * sum_i q_i (x - c_i)/|x - c_i|^3 at m points (SoA px/py/pz),
  * written to ex/ey/ez.  The same device code as the tracer's charge loop
  * (include/Electrostatics.h:38-60 summed over all charges). */

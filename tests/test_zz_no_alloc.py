@@ -1,0 +1,91 @@
+"""em_trace_dev_f32_noalloc: the allocation-free form of em_trace_dev_f32.
+
+Called through the C ABI, it must give the same rows and step counts as
+em_trace_dev_f32, bit for bit (the same kernels with the same launch shape),
+on two calls in a row where the second is smaller -- the fixed work array then
+holds the first call's contents -- and on the paths that need no work memory.
+Problems whose work memory exceeds the fixed array are refused before any
+launch, leaving the rows untouched.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import emtrace
+from emtrace import _lib, gen, rules
+
+pytestmark = pytest.mark.gpu
+
+LIMIT = 8 << 20  # EM_NOALLOC_SCRATCH_BYTES
+
+
+def _pair(gpu, w, rule, with_steps, pitch=None):
+    """Trace the same problem through em_trace_dev_f32 and the allocation-free form."""
+    c = torch.from_numpy(w.charges).cuda()
+    s = torch.from_numpy(w.seeds).cuda()
+    out = []
+    for noalloc in (False, True):
+        rows = gpu.seed_rows(s, w.len, pitch)
+        steps = torch.full((w.n,), -1, dtype=torch.int32, device="cuda") if with_steps else None
+        gpu.trace_into(c, rows, w.n, rule, steps=steps, noalloc=noalloc)
+        torch.cuda.synchronize()
+        out.append((rows[:, :, :w.n].cpu().numpy(), steps.cpu().numpy() if with_steps else None))
+    return out
+
+
+def _same(a, b):
+    (ra, sa), (rb, sb) = a, b
+    assert np.isfinite(ra).all()
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    if sa is not None:
+        assert np.array_equal(sa, sb)
+
+
+def test_noalloc_equals_trace_dev_on_two_calls(gpu):
+    """Line ends (angle rule with stop, >= 4,096 charges: the two-phase trace
+    over the fixed array), a larger call then a smaller one."""
+    rule = rules.angle()
+    big = gen.plain_config(6000, 4096, 20.0, 400, seed=31)
+    print("[noalloc] call 1: p 6000, n 4096, len 400, steps kept", flush=True)
+    a, b = _pair(gpu, big, rule, True)
+    _same(a, b)
+    assert (a[1] >= 1).all() and (a[1] > 192).any()  # the second phase ran
+    small = gen.plain_config(4200, 1500, 15.0, 260, seed=32)
+    print("[noalloc] call 2: p 4200, n 1500, len 260, steps in the array, pitch 1536",
+          flush=True)
+    a, b = _pair(gpu, small, rule, False, pitch=1536)
+    _same(a, b)
+
+
+def test_noalloc_paths_without_work_memory(gpu):
+    """Plain rule and line ends below 4,096 charges: no work memory, same rows."""
+    w = gen.plain_config(700, 900, 10.0, 120, seed=33)
+    for rule in (rules.plain(10.0), rules.angle()):
+        print(f"[noalloc] {rules.NAMES[rule.rule]} p 700 n 900 len 120", flush=True)
+        a, b = _pair(gpu, w, rule, True)
+        _same(a, b)
+
+
+def test_noalloc_refuses_beyond_the_fixed_array(gpu):
+    """200,000 lines need 48 B each of set-aside records: above 8 MiB, refused
+    with EM_ERR_INVALID before any launch (rows 1.. keep their sentinel)."""
+    n, length = 200_000, 2
+    assert 256 + 48 * n > LIMIT
+    w = gen.plain_config(4096, 64, 20.0, length, seed=34)
+    c = torch.from_numpy(w.charges).cuda()
+    seeds = torch.from_numpy(np.resize(w.seeds, (n, 3)).astype(np.float32)).cuda()
+    rows = gpu.seed_rows(seeds, length)
+    rows[:, 1:] = 7.0
+    print("[noalloc] refuse: p 4096 n 200000", flush=True)
+    with pytest.raises(_lib.EmError) as e:
+        gpu.trace_into(c, rows, n, rules.angle(), noalloc=True)
+    assert e.value.code == _lib.EM_ERR_INVALID
+    assert "fixed" in str(e.value)
+    torch.cuda.synchronize()
+    assert bool((rows[:, 1:] == 7.0).all())
+    # the same problem through the allocating form is accepted
+    gpu.trace_into(c, rows, n, rules.angle())
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(rows).all())
