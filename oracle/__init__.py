@@ -72,6 +72,9 @@ def lib() -> ctypes.CDLL:
         L.o64_field.argtypes = [P, S, P, S, ctypes.c_double, P]
         L.o64_field_jvp.argtypes = [P, S, P, S, ctypes.c_double, P, P]
         L.o64_trace.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams), ctypes.c_double, P, P]
+        L.o64_trace_jitter.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams), ctypes.c_double,
+                                       P, P, ctypes.c_double, ctypes.c_ulonglong]
+        L.o64_trace_jitter.restype = ctypes.c_int
         L.o64_adjoint.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams),
                                   ctypes.c_double, P, P, P]
         L.o64_adjoint_path.argtypes = [P, S, P, S, S, ctypes.POINTER(_StepParams),
@@ -233,6 +236,24 @@ def trace64(charges, seeds, length, rule, scale=None, with_steps=False):
                          ctypes.byref(sp), sc, _ptr(rows), _ptr(steps))
     if rc:
         raise ValueError(f"o64_trace rc={rc}")
+    return (rows, steps) if with_steps else rows
+
+
+def trace64_jitter(charges, seeds, length, rule, rel, key, scale=None, with_steps=False):
+    """trace64 with float32-sized rounding injected at every step: each field
+    component and each new coordinate is scaled by 1 +- rel, signs drawn from
+    ``key`` (deterministic).  Returns rows[len, n, 3] [, steps]."""
+    charges = np.ascontiguousarray(charges, dtype=np.float64)
+    seeds = np.ascontiguousarray(seeds, dtype=np.float64)
+    n = seeds.shape[0]
+    rows = np.zeros((length, n, 3), dtype=np.float64)
+    steps = np.zeros(n, dtype=np.uint32)
+    sp = _params(rule)
+    sc = rule.field_scale if scale is None else scale
+    rc = lib().o64_trace_jitter(_ptr(charges), charges.shape[0], _ptr(seeds), n, length,
+                                ctypes.byref(sp), sc, _ptr(rows), _ptr(steps), float(rel), int(key))
+    if rc:
+        raise ValueError(f"o64_trace_jitter rc={rc}")
     return (rows, steps) if with_steps else rows
 
 

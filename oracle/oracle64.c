@@ -13,6 +13,15 @@
  *                  derivative of the expression above).
  *   o64_trace      the plain (CPU_Implement.cpp:38-102) and angle step rules
  *                  in double with the physics-form field.
+ *   o64_trace_jitter the same with float32-sized relative perturbations of
+ *                  the field and the new point at every step (deterministic
+ *                  signs): the parity tests' measure of how far a line
+ *                  amplifies float32 rounding.  Pinned by: no perturbation
+ *                  -> o64_trace bit for bit; a single charge's straight lines
+ *                  stay within the injected size; the spread is linear in it.
+ *   o64_min_dist_f32 / o64_path_dist_f32  the parity metrics (nearest
+ *                  charge; distance to a reference polyline), pinned by
+ *                  brute force and known geometry.
  *   o64_adjoint    reverse-mode derivative of sum_s G_s . x_s through
  *                  o64_trace, w.r.t. every charge (x, y, z, q) and every seed.
  *                  No reference counterpart (the reference has no adjoint); it
@@ -140,10 +149,25 @@ static double step_rule(const o64_step_params* sp, int step, const double* E,
     return h;
 }
 
-/* rows: len*n*3, step-major, rows[(s*n + line)*3 + k]; row 0 = seeds. */
-int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
-              size_t len, const o64_step_params* sp, double scale, double* rows,
-              unsigned* steps)
+/* Relative perturbation +-rel, its sign from a SplitMix64 hash of (key, line,
+ * step, slot): the same draws on every run and thread count. */
+static double jitter(unsigned long long key, long long line, size_t step, int slot, double rel)
+{
+    unsigned long long z = key * 0x9E3779B97F4A7C15ull ^ ((unsigned long long)line << 20) ^
+                           ((unsigned long long)step << 3) ^ (unsigned long long)slot;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return 1.0 + ((z >> 63) ? rel : -rel);
+}
+
+/* rel > 0: every field component and every new coordinate is scaled by
+ * 1 +- rel at every step (signs from `key`) -- rounding of the size a
+ * float32 evaluation makes, injected into the FP64 path. */
+static int trace_impl(const double* ch, size_t p, const double* seeds, size_t n, size_t len,
+                      const o64_step_params* sp, double scale, double* rows, unsigned* steps,
+                      double rel, unsigned long long key)
 {
     if (!n) return 1;
     if (sp->rule == O64_REF) return 4; /* the REF rule is restated in restate.c */
@@ -162,12 +186,15 @@ int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
                 double E[3], u[3], L, c;
                 int br;
                 field64(ch, p, scale, x, E);
+                if (rel > 0)
+                    for (int k = 0; k < 3; ++k) E[k] *= jitter(key, line, s, k, rel);
                 double h = step_rule(sp, (int)s, E, up, hprev, u, &L, &c, &br);
                 if (reversed(sp, (int)s, u, up)) {
                     alive = 0;
                 } else {
                     for (int k = 0; k < 3; ++k) {
                         x[k] = x[k] + h * u[k];
+                        if (rel > 0) x[k] *= jitter(key, line, s, 3 + k, rel);
                         up[k] = u[k];
                     }
                     hprev = h;
@@ -179,6 +206,24 @@ int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
         if (steps) steps[line] = taken;
     }
     return 0;
+}
+
+/* rows: len*n*3, step-major, rows[(s*n + line)*3 + k]; row 0 = seeds. */
+int o64_trace(const double* ch, size_t p, const double* seeds, size_t n,
+              size_t len, const o64_step_params* sp, double scale, double* rows,
+              unsigned* steps)
+{
+    return trace_impl(ch, p, seeds, n, len, sp, scale, rows, steps, 0.0, 0ull);
+}
+
+/* The same FP64 trace with float32-sized rounding injected at every step (see
+ * trace_impl): the spread of such paths around o64_trace's measures how far
+ * a line amplifies the rounding any float32 implementation makes. */
+int o64_trace_jitter(const double* ch, size_t p, const double* seeds, size_t n,
+                     size_t len, const o64_step_params* sp, double scale, double* rows,
+                     unsigned* steps, double rel, unsigned long long key)
+{
+    return trace_impl(ch, p, seeds, n, len, sp, scale, rows, steps, rel, key);
 }
 
 /* Reverse-mode derivative of  L = sum_{s=0}^{len-1} G_s . x_s  (G: len*n*3,

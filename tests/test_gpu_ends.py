@@ -22,7 +22,7 @@ import pytest
 import torch
 
 from emtrace import gen, rules
-from helpers import parity_report, rows64_to_soa
+from helpers import line_gates, rows64_to_soa, step_gate
 from test_gpu_trace import dev
 
 pytestmark = pytest.mark.gpu
@@ -41,39 +41,15 @@ def escaping(p, n, half_side, length, seed):
     return ch, gen.make_seeds(ch, n, seed=seed)
 
 
-def check_lines(orc, g_rows, c_rows, charges, seeds, length, tag, rule=None):
-    """The angle rule's bar of the one-kernel path's tests, scaled for lines
-    that travel far: escaping lines reach |x| ~ 100, where one float32 ulp is
-    8e-6 and the CPU float32 path itself leaves the FP64 path, so fewer lines
-    qualify as well-conditioned (CPU float32 within 2e-5 of FP64, path
-    distance) -- at least 15 % here -- and on those, 99 % of GPU lines lie
-    within 1e-4 * max(1, R / 10) of the CPU path, R the line's largest
-    coordinate (1e-4 for lines within 10 of the origin).  Over all lines the
-    GPU's distance to FP64 is no worse than twice the CPU float32 path's
-    (median and 90th percentile)."""
+def check_lines(orc, g_rows, c_rows, charges, seeds, length, tag, rule=None, min_conditioned=0.15):
+    """The per-line gate of the one-kernel path's tests (tests/helpers.py,
+    line_gates) against the FP64 trajectory.  Escaping lines travel far (|x| up
+    to ~100, where one float32 ulp is 8e-6), so fewer lines are conditioned
+    than near the charges; at least 15 % must be."""
     rule = rule or rules.angle()
     r64 = rows64_to_soa(orc.trace64(charges, seeds, length, rule))
-    _, (_, path_g), _, _ = parity_report(g_rows, c_rows, charges, 1e-4)
-    _, (_, path_c64), _, _ = parity_report(c_rows, r64, charges, 1e-4)
-    _, (_, path_g64), _, _ = parity_report(g_rows, r64, charges, 1e-4)
-    cond = path_c64 <= 2e-5
-    print(f"[ends {tag}] conditioned {cond.sum()}/{len(cond)}; gpu-vs-cpu on them max "
-          f"{path_g[cond].max() if cond.any() else 0:.2e}; to FP64 median gpu "
-          f"{np.median(path_g64):.2e} cpu {np.median(path_c64):.2e}", flush=True)
-    assert np.isfinite(g_rows).all()
-    assert cond.mean() >= 0.15
-    R = np.nanmax(np.abs(c_rows), axis=(0, 1))
-    tol = 1e-4 * np.maximum(1.0, R / 10.0)
-    assert (path_g[cond] <= tol[cond]).mean() >= 0.99
-    assert np.median(path_g64) <= 2.0 * np.median(path_c64) + 1e-6
-    assert np.quantile(path_g64, 0.9) <= 2.0 * np.quantile(path_c64, 0.9) + 1e-6
-
-
-def same_steps(g_steps, c_steps):
-    """Step counts equal to the CPU path's on all but rounding-decided lines
-    (a direction reversal whose dot product is within rounding of zero)."""
-    same = g_steps == c_steps.astype(np.int64)
-    return same.mean() >= 0.97
+    return line_gates(f"ends {tag}", g_rows, c_rows, r64, charges, seeds, rule,
+                      min_conditioned=min_conditioned)
 
 
 def ends_trace(gpu, charges, seeds, length, launch=None, pitch=None, rule=None):
@@ -116,7 +92,7 @@ def test_across_the_cap_vs_cpu_path(gpu, orc):
     print(f"[ends] lines_64k subset/600: lines past the cap {past}/{len(g_steps)}, "
           f"identical steps {(g_steps == c_steps).sum()}", flush=True)
     assert past >= 16  # the second phase ran
-    assert (g_steps == c_steps.astype(np.int64)).mean() >= 0.99
+    step_gate("ends lines_64k subset/600", w.charges, g_rows, g_steps, c_rows, c_steps)
     check_lines(orc, g_rows, c_rows, w.charges, seeds, length, "lines_64k subset/600")
     # rows after a line's end repeat its end point
     for l in range(len(g_steps)):
@@ -191,7 +167,7 @@ def test_charge_counts_vs_cpu_path(gpu, orc, p):
     g_rows, g_steps = ends_trace(gpu, ch, seeds, length, rule=SHORT_STEPS)
     rc, c_rows, c_steps = orc.calc_lines(ch, seeds, length, SHORT_STEPS, with_steps=True)
     assert rc == 0
-    assert same_steps(g_steps, c_steps)
+    step_gate(f"ends p={p}", ch, g_rows, g_steps, c_rows, c_steps)
     # dense positive charges can still turn a line back next to its seed
     assert (g_steps == length - 1).mean() >= 0.8
     if p == 1:  # radial straight lines: pointwise
@@ -212,7 +188,7 @@ def test_many_lines_past_the_cap(gpu, orc, lines):
     g_rows, g_steps = ends_trace(gpu, ch, seeds, length, rule=SHORT_STEPS)
     rc, c_rows, c_steps = orc.calc_lines(ch, seeds, length, SHORT_STEPS, with_steps=True)
     assert rc == 0
-    assert same_steps(g_steps, c_steps)
+    step_gate(f"ends {lines} escaping", ch, g_rows, g_steps, c_rows, c_steps)
     past = (g_steps > CAP).sum()
     assert past >= (2000 if lines == 3000 else TAIL_SLOTS + 1)
     check_lines(orc, g_rows, c_rows, ch, seeds, length, f"{lines} escaping", SHORT_STEPS)
@@ -225,7 +201,7 @@ def test_lengths_at_the_cap(gpu, orc, length):
     g_rows, g_steps = ends_trace(gpu, ch, seeds, length)
     rc, c_rows, c_steps = orc.calc_lines(ch, seeds, length, rules.angle(), with_steps=True)
     assert rc == 0
-    assert same_steps(g_steps, c_steps)
+    step_gate(f"ends len={length}", ch, g_rows, g_steps, c_rows, c_steps)
     if length > 2:
         check_lines(orc, g_rows, c_rows, ch, seeds, length, f"len={length}")
     else:
@@ -241,7 +217,7 @@ def test_no_charges(gpu, orc):
     g_rows, g_steps = ends_trace(gpu, empty, w.seeds, length)
     rc, c_rows, c_steps = orc.calc_lines(empty, w.seeds, length, rules.angle(), with_steps=True)
     assert rc == 0
-    assert same_steps(g_steps, c_steps)
+    step_gate("ends no charges", empty, g_rows, g_steps, c_rows, c_steps)
     assert np.array_equal(g_rows[:, 0], c_rows[:, 0])
     assert np.isnan(g_rows[:, 1:]).all() and np.isnan(c_rows[:, 1:]).all()
 

@@ -1,15 +1,19 @@
 """GPU parity tests of the forward tracer (HIP path through the C ABI).
 
-Parity bar (float32 trajectories):
-  * plain / ref rules: GPU points vs the CPU path at the same step index;
-    angle rule: distance of GPU points to the CPU path's polyline (the
-    adaptive step makes along-line position an ill-conditioned function of
-    rounding, see tests/helpers.py);
-  * lines whose CPU float32 path tracks the FP64 path within 2e-5 must lie
-    within 1e-4 of the CPU path; over all lines, the GPU's distance to FP64
-    must be no worse than the CPU float32 path's (median and 90th percentile);
-  * the literal rule -- every point within 1e-4 for lines that stay more than
-    0.05 from any charge -- is reported for every case.
+Parity bar (float32 trajectories), applied line by line (tests/helpers.py,
+``line_gates``): over each line's prefix -- before its first approach within
+0.05 of a charge, the whole line for a line that stays farther -- the GPU
+points lie within 1e-4 of the CPU path (pointwise for the plain and ref
+rules; distance to the CPU path's polyline for the angle rule, whose adaptive
+step makes the along-line position an ill-conditioned function of rounding),
+unless that line is shown to be beyond float32 resolution -- the CPU float32
+path departs from FP64 by more than 2e-5 there, or the FP64 path itself moves
+by more than 2e-5 under float32-sized perturbations -- and the GPU-CPU gap is
+within 10x that departure or move.  Each exemption is printed with its
+numbers.  Over all lines the GPU's distance to FP64 is no
+worse than the CPU float32 path's (median and 90th percentile, 2x).
+Step counts: len - 1 on every line without the stop rule; with it, identical
+to the CPU path's or decided by rounding at the deciding step (``step_gate``).
 Field evaluations: within 1e-5 relative of the FP64 field where the field is
 not dominated by cancellation (|E| >= 0.1 * sum_i |E_i|).
 """
@@ -22,7 +26,8 @@ import torch
 
 import oracle
 from emtrace import gen, rules
-from helpers import parity_report, rows64_to_soa
+from helpers import (COND, TOL, line_errors, line_gates, parity_report, prefix_lengths,
+                     rows64_to_soa, step_gate)
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -45,70 +50,94 @@ def print_report(tag, rep):
                                          else f"{k}={rep[k]}" for k in keys), flush=True)
 
 
-def check_against_cpu(tag, g_rows, c_rows, r64, charges, rule, tol=1e-4):
-    rep, (perr_g, path_g), first, eligible = parity_report(g_rows, c_rows, charges, tol)
-    print_report(tag + " gpu-vs-cpu", rep)
-    rep_c, (perr_c, path_c), _, _ = parity_report(c_rows, r64, charges, tol)
-    rep_g, (perr_g64, path_g64), _, _ = parity_report(g_rows, r64, charges, tol)
-    print_report(tag + " cpu-vs-fp64", rep_c)
-    print_report(tag + " gpu-vs-fp64", rep_g)
-    use_path = rule.rule == rules.ANGLE
-    e_gc = path_g if use_path else perr_g
-    e_c64 = path_c if use_path else perr_c
-    e_g64 = path_g64 if use_path else perr_g64
-    conditioned = e_c64 <= 2e-5
-    print(f"[parity] {tag}: conditioned lines {conditioned.sum()}/{len(conditioned)}; "
-          f"max gpu-vs-cpu on them {e_gc[conditioned].max() if conditioned.any() else 0:.3g}", flush=True)
-    assert conditioned.mean() > 0.5
-    assert (e_gc[conditioned] <= tol).mean() >= 0.99
-    assert np.median(e_g64) <= 2.0 * np.median(e_c64) + 1e-6
-    assert np.quantile(e_g64, 0.9) <= 2.0 * np.quantile(e_c64, 0.9) + 1e-6
-    return rep
-
-
 def _golden_f32():
     return sorted(p for p in os.listdir(GOLDEN) if p.startswith("ref_f32"))
 
 
 @pytest.mark.parametrize("name", _golden_f32())
-def test_gpu_matches_reference_golden(gpu, name):
-    """Reference CalcField_CPU<float> output (compiled reference) vs the HIP path."""
+def test_gpu_matches_reference_golden(gpu, orc, name):
+    """Reference CalcField_CPU<float> output (compiled reference, golden
+    vectors) vs the HIP path, line by line.  The tolerances scale with the
+    step length (1e-4 and 2e-5 at step 0.1): over the same number of rows a
+    longer step carries the direction's rounding proportionally farther."""
     g = np.load(os.path.join(GOLDEN, name))
     res = float(g["resolution"])
     rule = rules.ref(res) if int(g["use_curvature"]) else rules.plain(res)
-    rows = gpu_trace(gpu, g["charges"], g["seeds"], g["rows"].shape[1], rule)
+    length = g["rows"].shape[1]
+    rows, steps = gpu.trace(dev(g["charges"]), dev(g["seeds"]), length, rule, with_steps=True)
+    rows = rows.cpu().numpy()
     assert np.array_equal(rows[:, 0], g["rows"][:, 0])
-    assert np.isfinite(rows).all()
-    step = 1.0 / res
-    rep, (perr, _), first, _ = parity_report(rows, g["rows"], g["charges"], tol=1e-4 * step / 0.1)
-    print_report(name, rep)
-    assert rep["prefix_median_err"] <= 1e-5 * step / 0.1
-    assert rep["prefix_within_tol"] >= 0.95
+    assert (steps.cpu().numpy() == length - 1).all()
+    rc, r64 = orc.calc_lines(g["charges"], g["seeds"], length, rules.plain(res), dtype=np.float64)
+    assert rc == 0
+    k = (1.0 / res) / 0.1
+    line_gates(name, rows, g["rows"], r64, g["charges"], g["seeds"], rule, tol=TOL * k,
+               cond=COND * k, min_conditioned=0.8)
 
 
-@pytest.mark.parametrize("rule", [rules.ref(10.0), rules.plain(10.0), rules.angle(),
-                                  rules.angle(stop=False)],
-                         ids=["ref", "plain", "angle", "angle-nostop"])
-def test_lines_8k_vs_cpu_path(gpu, orc, rule):
-    """lines_8k charges (1,000 in [-10,10]^3), 512 of its 8,192 lines, 2,500 rows."""
+_FP64_ROWS = {}
+
+
+def _lines_8k_fp64(orc, w, rule):
+    """FP64 rows of all lines_8k lines (the restatement in double for plain
+    and ref -- (k+1) == 1 in double as in float -- the FP64 trajectory for the
+    angle rule), computed once per rule."""
+    key = (rule.rule == rules.ANGLE, rule.stop)
+    if key not in _FP64_ROWS:
+        if rule.rule == rules.ANGLE:
+            _FP64_ROWS[key] = rows64_to_soa(orc.trace64(w.charges, w.seeds, w.len, rule))
+        else:
+            rc, r64 = orc.calc_lines(w.charges, w.seeds, w.len, rules.plain(rule.resolution),
+                                     dtype=np.float64)
+            assert rc == 0
+            _FP64_ROWS[key] = r64
+    return _FP64_ROWS[key]
+
+
+@pytest.mark.parametrize("rule", [rules.ref(10.0), rules.plain(10.0)], ids=["ref", "plain"])
+def test_full_lines_8k_vs_compiled_reference(gpu, orc, rule):
+    """All of lines_8k (1,000 charges, 8,192 lines, 2,500 rows, resolution 10)
+    against the reference's own CalcField_CPU<float> (CPU_Implement.cpp:
+    207-351 with useCurvature, 38-102 without), compiled from source: every
+    line takes len - 1 steps, and every line passes the per-line gate."""
+    if oracle.ref_lib() is None:
+        pytest.skip("compiled reference absent (oracle/_ref)")
     w = gen.lines_8k()
-    idx = np.arange(0, w.n, 16)
-    seeds = w.seeds[idx]
-    rows_t, steps_t = gpu.trace(dev(w.charges), dev(seeds), w.len, rule, with_steps=True)
+    rows_t, steps_t = gpu.trace(dev(w.charges), dev(w.seeds), w.len, rule, with_steps=True)
+    torch.cuda.synchronize()
+    g_rows, g_steps = rows_t.cpu().numpy(), steps_t.cpu().numpy()
+    del rows_t
+    curv = int(rule.rule == rules.REF)
+    rc, c_rows, t_ref, _ = orc.ref_calc_lines(w.charges, w.seeds, w.len, rule.resolution, curv)
+    assert rc == 0
+    print(f"[full lines_8k] {rules.NAMES[rule.rule]}: compiled reference {t_ref:.2f} s", flush=True)
+    assert (g_steps == w.len - 1).all()
+    r64 = _lines_8k_fp64(orc, w, rule)
+    line_gates(f"lines_8k all/{rules.NAMES[rule.rule]} vs reference", g_rows, c_rows, r64,
+               w.charges, w.seeds, rule, min_conditioned=0.85)
+
+
+@pytest.mark.parametrize("rule", [rules.angle(), rules.angle(stop=False)],
+                         ids=["angle", "angle-nostop"])
+def test_full_lines_8k_angle_vs_cpu_path(gpu, orc, rule):
+    """All of lines_8k with the angle rule (no reference counterpart) against
+    the C restatement of the CPU path in the HIP code's operation order: step
+    counts per line (step_gate), points per line (path distance)."""
+    w = gen.lines_8k()
+    rows_t, steps_t = gpu.trace(dev(w.charges), dev(w.seeds), w.len, rule, with_steps=True)
     torch.cuda.synchronize()
     g_rows, g_steps = rows_t.cpu().numpy(), steps_t.cpu().numpy().astype(np.int64)
-    rc, c_rows, c_steps = orc.calc_lines(w.charges, seeds, w.len, rule, with_steps=True)
+    del rows_t
+    rc, c_rows, c_steps = orc.calc_lines(w.charges, w.seeds, w.len, rule, with_steps=True)
     assert rc == 0
-    r64_rule = rules.plain(rule.resolution) if rule.rule == rules.REF else rule
-    r64 = rows64_to_soa(orc.trace64(w.charges, seeds, w.len, r64_rule))
-    assert np.isfinite(g_rows).all()
-    same = g_steps == c_steps.astype(np.int64)
-    print(f"[steps] lines_8k/{rules.NAMES[rule.rule]}: identical step counts {same.sum()}/{len(same)}; "
-          f"mean steps {g_steps.mean():.1f} (stop={rule.stop})", flush=True)
-    if not rule.stop:
-        assert (g_steps == w.len - 1).all()
-    assert same.mean() >= 0.99
-    check_against_cpu(f"lines_8k/{rules.NAMES[rule.rule]}", g_rows, c_rows, r64, w.charges, rule)
+    tag = f"lines_8k all/{rules.NAMES[rule.rule]}{'' if rule.stop else '-nostop'}"
+    if rule.stop:
+        print(f"[steps] {tag}: mean steps {g_steps.mean():.1f}", flush=True)
+        step_gate(tag, w.charges, g_rows, g_steps, c_rows, c_steps)
+    else:
+        assert (g_steps == w.len - 1).all() and (c_steps == w.len - 1).all()
+    r64 = _lines_8k_fp64(orc, w, rule)
+    line_gates(tag, g_rows, c_rows, r64, w.charges, w.seeds, rule, min_conditioned=0.5)
 
 
 def test_field_matches_fp64(gpu, orc):
@@ -154,7 +183,12 @@ def test_ref_rule_bitwise_equals_plain(gpu):
     a = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.plain(10.0))
     b = gpu_trace(gpu, w.charges, w.seeds, w.len, rules.ref(10.0))
     same = (a == b).all(axis=(0, 1))
-    assert same.mean() > 0.99, same.mean()
+    print(f"[ref==plain] identical lines {same.sum()}/{len(same)}", flush=True)
+    # on every line, up to its first approach within 0.05 of a charge (closer,
+    # |(E-P) x P|^2 can overflow float and the ref rule's k is no longer 0)
+    first, _ = prefix_lengths(a, w.charges)
+    for l in range(w.n):
+        assert np.array_equal(a[:, :first[l], l], b[:, :first[l], l]), l
 
 
 @pytest.mark.parametrize("S", [1, 2, 4, 8, 16, 32])
@@ -238,10 +272,9 @@ EDGE = [
 @pytest.mark.parametrize("case", EDGE, ids=[c[0] for c in EDGE])
 def test_edge_shapes_vs_cpu_path(gpu, orc, case):
     """Shapes at the edges of the work split and of the charge tiles, against
-    the CPU path: same step counts, points within 1e-4 before the first
-    approach within 0.05 of a charge (REF needs n % 64 on the CPU path: lines
-    are padded there and cut)."""
-    from helpers import line_errors, prefix_lengths
+    the CPU path: the same step count on every line, every line through the
+    per-line gate (REF needs n % 64 on the CPU path: lines are padded there
+    and cut)."""
     name, p, n, length, rule, shape = case
     w = gen.plain_config(p, n, 5.0, length, seed=21)
     launch = gpu.make_launch(shape[0], shape[1], shape[2]) if shape else None
@@ -254,11 +287,17 @@ def test_edge_shapes_vs_cpu_path(gpu, orc, case):
     assert rc == 0
     c_rows, c_steps = c_rows[:, :, :n], c_steps[:n]
     assert np.array_equal(steps.cpu().numpy().astype(np.int64), c_steps.astype(np.int64))
+    if rule.rule == rules.ANGLE:
+        r64 = rows64_to_soa(orc.trace64(w.charges, w.seeds, length, rule))
+    else:
+        rc, r64 = orc.calc_lines(w.charges, w.seeds, length, rules.plain(rule.resolution),
+                                 dtype=np.float64)
+        assert rc == 0
     first, _ = prefix_lengths(c_rows, w.charges)
     err = line_errors(rows, c_rows, first)
     print(f"[edge {name}] lines {n}: median {np.median(err):.2e} max {err.max():.2e}", flush=True)
-    assert np.isfinite(rows).all()
-    assert (err <= 1e-4).mean() >= 0.95 and np.median(err) <= 1e-5
+    assert np.median(err) <= 1e-5
+    line_gates(f"edge {name}", rows, c_rows, r64, w.charges, w.seeds, rule, min_conditioned=0.3)
 
 
 def test_tile_beyond_the_lds_budget_is_rejected(gpu):

@@ -479,3 +479,44 @@ def test_trace64_plain_matches_reference_double_rows(orc):
     perr = np.array([err[: first[l], l].max() if first[l] else 0.0 for l in range(ref.shape[2])])
     assert (first > 10).mean() > 0.5
     assert perr.max() < 1e-8, perr.max()
+
+
+def test_trace64_jitter_pins(orc):
+    """o64_trace_jitter: with no injected rounding it is o64_trace bit for bit;
+    the same key repeats, another key differs; on one charge's radial lines the
+    injected +-2^-24 at every step moves the path by no more than the sum of
+    what was injected (the field of a point charge neither pulls a line back
+    nor pushes it away from the radial through its point); and the move is
+    linear in the injected size on a smooth many-charge line set."""
+    w = gen.plain_config(40, 16, 4.0, 80, seed=3)
+    for rule in (rules.plain(10.0), rules.angle()):
+        a, sa = orc.trace64(w.charges, w.seeds, w.len, rule, with_steps=True)
+        b, sb = orc.trace64_jitter(w.charges, w.seeds, w.len, rule, 0.0, 7, with_steps=True)
+        assert np.array_equal(a, b) and np.array_equal(sa, sb)
+        j1 = orc.trace64_jitter(w.charges, w.seeds, w.len, rule, 2.0 ** -24, 1)
+        assert np.array_equal(j1, orc.trace64_jitter(w.charges, w.seeds, w.len, rule, 2.0 ** -24, 1))
+        assert not np.array_equal(j1, orc.trace64_jitter(w.charges, w.seeds, w.len, rule, 2.0 ** -24, 2))
+    # one charge: radial lines of 60 steps of 0.1 from r0 = 0.1
+    c = np.array([[0.5, -0.25, 1.0, 1.0]])
+    d = gen.fibonacci_dirs(32)
+    seeds = c[0, :3] + 0.1 * d
+    rel = 2.0 ** -24
+    base = orc.trace64(c, seeds, 60, rules.plain(10.0))
+    jit = orc.trace64_jitter(c, seeds, 60, rules.plain(10.0), rel, 3)
+    dev = np.abs(jit - base).max(axis=2)                     # (len, n)
+    assert dev[1:].min() > 0
+    # per step: the direction by <= sqrt(3) rel (moving the point h times that)
+    # and each coordinate by rel |x_k|; the largest |x_k| is <= |c_k| + r
+    reach = np.abs(c[0, :3]).max() + 0.1 + 0.1 * np.arange(60)
+    bound = np.cumsum(np.sqrt(3) * rel * 0.1 + rel * reach)
+    assert (dev <= bound[:, None] * 1.01).all()
+    # linear in the injected size (the perturbation stays in the linear regime)
+    base = orc.trace64(w.charges, w.seeds, w.len, rules.plain(10.0))
+    m1 = np.abs(orc.trace64_jitter(w.charges, w.seeds, w.len, rules.plain(10.0), 1e-9, 5) - base)
+    m2 = np.abs(orc.trace64_jitter(w.charges, w.seeds, w.len, rules.plain(10.0), 2e-9, 5) - base)
+    first, _ = prefix_lengths(rows64_to_soa(base).astype(np.float32), w.charges)
+    for l in range(w.n):
+        f = first[l]
+        if f > 5:
+            r = m2[:f, l].max() / m1[:f, l].max()
+            assert abs(r - 2.0) < 0.05, (l, r)
