@@ -18,7 +18,8 @@ source into oracle/_ref), or the C restatement in oracle/ where that build is
 absent, on a bounded slice of the same workload.
 
 Extra fields (skipped with --no-extras): the same workload with line ends at
-sinks (useful interactions only), the batched workload (512 configurations,
+sinks (useful interactions only), lines_8k in full on the GPU and through the
+reference's CPU path, the batched workload (512 configurations,
 one hipGraph-replayed persistent launch), the differentiation workload
 (forward + adjoint time over forward time), the inverse fit, the FP64 kernel,
 the reference application's default problem and the particle step.
@@ -56,7 +57,7 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extras", default="all",
                     help="comma-separated extra workloads to run (default: all): lines_64k_with_line_ends, "
-                         "batch_512, adjoint_2k, fit_64, fp64_kernel, "
+                         "lines_8k, batch_512, adjoint_2k, fit_64, fp64_kernel, "
                          "legacy_default_preset, particle_step")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH/WRITE_SIZE counters)")
@@ -83,13 +84,41 @@ def timed(fn, steps, warmup):
     return wall, [a.elapsed_time(b) * 1e-3 for a, b in evs]
 
 
+def host_cpu():
+    """The host CPU as the baseline runs on it: model name, CPUs this process
+    may run on (affinity), CPUs in the machine, and the OpenMP thread count a
+    parallel region gets (what the reference's OpenMP loop uses)."""
+    import ctypes
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        omp = int(ctypes.CDLL("libgomp.so.1").omp_get_max_threads())
+    except OSError:
+        omp = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    affinity = len(os.sched_getaffinity(0))
+    info = {"cpu_model": model, "omp_threads": omp, "affinity_cpus": affinity,
+            "machine_cpus": os.cpu_count()}
+    if omp < affinity:
+        info["threads_note"] = (f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')} caps the "
+                                f"OpenMP team below the {affinity} CPUs this process may use")
+    return info
+
+
 def cpu_baseline(w, lines):
     """Time the reference (or the restatement) on `lines` lines of the workload."""
     import numpy as np
     sys.path.insert(0, ROOT)
     import oracle  # the checker / CPU baseline only
     from emtrace import rules
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    host = host_cpu()
+    threads = host["omp_threads"]
     lines = max(64, lines // 64 * 64)
     seeds = w.seeds[np.linspace(0, w.n - 1, lines).astype(np.int64)]
     t0 = time.perf_counter()
@@ -108,7 +137,7 @@ def cpu_baseline(w, lines):
     return {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": kind,
             "sample": f"{path}: {w.p} charges x {lines} of {w.n} lines x {w.len} rows "
                       f"(every line all {w.len - 1} steps, step 0.1), {dt:.2f} s wall; cost is "
-                      f"linear in the line count", "seconds": dt}
+                      f"linear in the line count", "seconds": dt, **host}
 
 
 def copy_only():
@@ -266,6 +295,34 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                     "the lines still moving, up to 8 at a time; rows after a line's end are then "
                     "filled"}
         del rows
+    # 1b. lines_8k (1,000 charges in [-10,10]^3, 8,192 lines, 2,500 rows) in full:
+    #     the GPU kernel (ref rule, resolution 10: the reference's live default
+    #     step; and the angle rule, every line all steps) and the reference's
+    #     own CalcField_CPU<float> with useCurvature on all 8,192 lines
+    if on("lines_8k"):
+        w0 = gen.lines_8k()
+        c0, s0 = torch.from_numpy(w0.charges).cuda(), torch.from_numpy(w0.seeds).cuda()
+        r0 = emtrace.seed_rows(s0, w0.len)
+        rec = {"charges": w0.p, "lines": w0.n, "rows": w0.len, "interactions": float(w0.interactions)}
+        for name, rule0 in (("ref", rules.ref(10.0)), ("angle", rules.angle(stop=False))):
+            _, k0 = timed(lambda: emtrace.trace_into(c0, r0, w0.n, rule0), 3, 1)
+            rec[f"gpu_{name}_kernel_ms"] = min(k0) * 1e3
+            rec[f"gpu_{name}_interactions_per_s"] = float(w0.interactions) / min(k0)
+        del r0
+        sys.path.insert(0, ROOT)
+        import oracle  # the CPU baseline only
+        if oracle.ref_lib() is not None:
+            t0 = time.perf_counter()
+            rc, _, t_in, _ = oracle.ref_calc_lines(w0.charges, w0.seeds, w0.len, 10.0, 1)
+            dt = time.perf_counter() - t0
+            if rc == 0:
+                rec["cpu_reference_s"] = dt
+                rec["cpu_reference_interactions_per_s"] = float(w0.interactions) / dt
+                rec["gpu_ref_speedup"] = dt / (rec["gpu_ref_kernel_ms"] * 1e-3)
+                rec["cpu_note"] = ("reference CalcField_CPU<float>, useCurvature=true (SSE + "
+                                   "OpenMP), all 8,192 lines, wall time of the call")
+                rec.update({k: v for k, v in host_cpu().items()})
+        res["lines_8k"] = rec
     # 2. batched workload (512 configurations, 1,000 rows), one graph-replayed launch
     if on("batch_512"):
         ws = gen.batch_512()
