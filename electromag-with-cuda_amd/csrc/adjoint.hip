@@ -159,8 +159,9 @@ __device__ __forceinline__ void field_tensor_at(float4* __restrict__ tile, uint3
 
 // Recompute record of a segment: five float4 per step, one set per line (the
 // S lanes of a line hold the same values; the group's first lane writes them
-// and all S read them): rec[(j*5 + q) * lanes + slot], slot = the group's first
-// lane, so a wave's accesses stay within one 1 KiB run.
+// and all S read them): rec[(j*5 + q) * groups + slot], slot = the line
+// group's index in the grid and groups = the grid's lanes / S, so a wave's
+// 64 / S slots are consecutive records.
 constexpr int kRec = 5;
 struct RecPtr {
     float4* base;
@@ -195,7 +196,7 @@ k_adjoint_lines(const Cfg* __restrict__ cfgs, const Item* __restrict__ items,
         stage(tile, ch, p, pp);
         __syncthreads();
     }
-    RecPtr rec{rec_base, (uint64_t)gridDim.x * blockDim.x, (uint64_t)blockIdx.x * blockDim.x + tid - k};
+    RecPtr rec{rec_base, (uint64_t)gridDim.x * (blockDim.x / S), ((uint64_t)blockIdx.x * blockDim.x + tid) / S};
     const uint32_t steps = cfg.len - 1;
     const uint32_t nseg = (steps + every - 1) / every;
     float lx = 0.f, ly = 0.f, lz = 0.f;     // dL/dx carried backward
@@ -596,10 +597,9 @@ struct AdjPlan {
     float4* d_part = nullptr;
     size_t part_bytes = 0;
     // single-configuration calls: the descriptor the work lists were built
-    // for, and the stream and completion event of the last call's launches
+    // for, and the completion event of the last call's launches
     Cfg single{};
     bool single_ok = false;
-    hipStream_t last = nullptr;
     hipEvent_t done = nullptr;
 };
 
@@ -673,9 +673,10 @@ int adj_prepare(AdjPlan* a, const std::vector<Cfg>& cfgs, const Shape& sh, uint3
         part += (uint64_t)chunks * c.p;
     }
     a->w_plane = line_elems;
-    const uint64_t lanes = (uint64_t)a->line_items.size() * sh.block;
+    // one record set per line group of the lines pass's grid (see RecPtr)
+    const uint64_t groups = (uint64_t)a->line_items.size() * (uint64_t)(sh.block / sh.S);
     EM_TRY(grow(reinterpret_cast<void**>(&a->d_rec), &a->rec_bytes,
-                lanes * (size_t)every * kRec * sizeof(float4)));
+                groups * (size_t)every * kRec * sizeof(float4)));
     size_t wb = 0;
     if (a->d_w) wb = 0;  // always re-size below
     if (a->d_w) EM_HIP(hipFree(a->d_w));
@@ -790,8 +791,9 @@ extern "C" int em_adjoint_dev_f32(const float* d_charges, size_t p, const float*
     cfg.len = (uint32_t)len;
     cfg.grad_line_off = (uint32_t)n;  // seed-gradient plane stride
     // The work lists and buffers are rebuilt only for a new problem or shape;
-    // a repeated call reuses them without host synchronization (on another
-    // stream it first waits, on the device, for the last call's launches).
+    // a repeated call reuses them without host synchronization: it first
+    // waits, on the device, for the last call's launches (always: a stream
+    // handle equal to the last one may be a new stream in its place).
     const bool same = a->single_ok && a->device == dev.device && a->every == ckpt_every &&
                       a->sh.S == sh.S && a->sh.block == sh.block && a->sh.tile == sh.tile &&
                       a->sh.unroll == sh.unroll && a->single.pitch == cfg.pitch &&
@@ -807,14 +809,13 @@ extern "C" int em_adjoint_dev_f32(const float* d_charges, size_t p, const float*
         if (!a->done) EM_HIP(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
         a->single = cfg;
         a->single_ok = true;
-    } else if (a->last != st) {
+    } else {
         EM_HIP(hipStreamWaitEvent(st, a->done, 0));
     }
     EM_TRY(adj_launch(a, a->d_cfgs_own, reinterpret_cast<const float4*>(d_charges), d_x, d_y, d_z,
                       d_ckpt, d_gx, d_gy, d_gz, to_stepp(*step),
                       reinterpret_cast<float4*>(d_grad_charges), d_grad_seeds, st));
     EM_HIP(hipEventRecord(a->done, st));
-    a->last = st;
     return EM_OK;
 }
 

@@ -301,7 +301,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
 {
     const double t0 = now_s();
     if (perf) {
-        perf->progress = 0.0;
+        set_progress(perf, 0.0);
         perf->time_s = perf->kernel_s = perf->h2d_s = perf->d2h_s = 0.0;
         perf->h2d_bytes = perf->d2h_bytes = 0.0;
         perf->gflops = 0.0;
@@ -378,7 +378,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
         const double td = now_s();
         EM_TRY(d2h_staged(dev.device, segs, (steps && c + 1 == chunks) ? 4 : 3, b.copy));
         d2h += now_s() - td;
-        if (perf) perf->progress = (double)bound(c + 1) / (double)len;
+        set_progress(perf, (double)bound(c + 1) / (double)len);
     }
     EM_HIP(hipEventSynchronize(b.ev[2]));
 
@@ -398,7 +398,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
         perf->tile = (int32_t)sh.tile;
         perf->block = sh.block;
         perf->unroll = sh.unroll;
-        perf->progress = 1.0;
+        set_progress(perf, 1.0);
     }
     return EM_OK;
 }

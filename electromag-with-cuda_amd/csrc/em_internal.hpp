@@ -81,6 +81,14 @@ int validate_shape(const DeviceInfo& dev, const Shape& s);
 // waves per SIMD, whether the charges are re-staged, and line ends).
 int default_unroll(const DeviceInfo& dev, const Shape& s, uint64_t p, uint64_t n, bool stop);
 
+// perf->progress = v, mirrored into *perf->progress_out when the caller gave one
+inline void set_progress(em_perf* perf, double v)
+{
+    if (!perf) return;
+    perf->progress = v;
+    if (perf->progress_out) *perf->progress_out = v;
+}
+
 StepP to_stepp(const em_step& st);
 int validate_step(const em_step* st);
 

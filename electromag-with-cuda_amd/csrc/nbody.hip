@@ -285,7 +285,7 @@ int em_verlet_step_f32(float* particles, size_t n, const float* charges, size_t 
     clear_error();
     const auto t0 = std::chrono::steady_clock::now();
     if (perf) {
-        perf->progress = 0.0;
+        set_progress(perf, 0.0);
         perf->time_s = perf->kernel_s = perf->h2d_s = perf->d2h_s = 0.0;
     }
     EM_TRY(check_verlet(n, p, box_min, box_max, dt));
@@ -330,7 +330,7 @@ int em_verlet_step_f32(float* particles, size_t n, const float* charges, size_t 
         perf->time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         perf->interactions = (double)n * ((double)p + (double)n - 1.0);
         perf->interactions_per_s = perf->kernel_s > 0 ? perf->interactions / perf->kernel_s : 0.0;
-        perf->progress = 1.0;
+        set_progress(perf, 1.0);
     }
     return EM_OK;
 }

@@ -271,7 +271,7 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
 {
     const double t0 = now64();
     if (perf) {
-        perf->progress = 0.0;
+        set_progress(perf, 0.0);
         perf->time_s = perf->kernel_s = perf->h2d_s = perf->d2h_s = 0.0;
         perf->h2d_bytes = perf->d2h_bytes = perf->gflops = 0.0;
     }
@@ -344,7 +344,7 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
         perf->tile = (int32_t)(p <= kTile64 ? std::max<size_t>(p, 1) : kTile64 / 2);
         perf->block = kBlock64;
         perf->unroll = 0;  // the FP64 kernel's loop has no pipelined groups
-        perf->progress = 1.0;
+        set_progress(perf, 1.0);
     }
     return EM_OK;
 }

@@ -53,10 +53,11 @@ class em_perf(ctypes.Structure):
                 ("interactions", ctypes.c_double), ("interactions_per_s", ctypes.c_double),
                 ("line_steps_per_s", ctypes.c_double), ("gflops", ctypes.c_double),
                 ("progress", ctypes.c_double), ("lanes_per_line", ctypes.c_int32),
-                ("tile", ctypes.c_int32), ("block", ctypes.c_int32), ("unroll", ctypes.c_int32)]
+                ("tile", ctypes.c_int32), ("block", ctypes.c_int32), ("unroll", ctypes.c_int32),
+                ("progress_out", ctypes.POINTER(ctypes.c_double))]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "progress_out"}
 
 
 class em_batch_cfg(ctypes.Structure):

@@ -309,6 +309,53 @@ class BatchLayout:
         return int((self.p * self.n * (self.len - 1)).sum())
 
 
+def check_batch_buffers(layout: BatchLayout, charges, lines, *, steps=None, ckpt=None,
+                        ckpt_floats: Optional[int] = None, grad_lines=None, grad_charges=None,
+                        on_device: bool = True) -> None:
+    """Validate the packed buffers of a batch call before any launch: float32
+    (int32 for steps), contiguous, and at least as large as the layout needs
+    -- charges (>= sum p, 4), lines and grad_lines (3, >= total line elements),
+    steps (>= sum n), ckpt (>= ckpt_floats), grad_charges (>= sum p, 4).
+    Raises TypeError / ValueError.  ``on_device`` also requires GPU tensors."""
+    torch = _torch()
+
+    def tensor(t, name, dtype):
+        if on_device:
+            _dev_check(t, name, dtype)
+        else:
+            if not isinstance(t, torch.Tensor):
+                raise TypeError(f"{name} must be a torch tensor")
+            if not t.is_contiguous():
+                raise ValueError(f"{name} must be contiguous")
+            if t.dtype != dtype:
+                raise TypeError(f"{name} must be {dtype}")
+
+    def records(t, name):
+        tensor(t, name, torch.float32)
+        if t.dim() != 2 or t.shape[1] != 4 or t.shape[0] < layout.total_charges:
+            raise ValueError(f"{name} must be (>= {layout.total_charges}, 4), got {tuple(t.shape)}")
+
+    def planes(t, name):
+        tensor(t, name, torch.float32)
+        if t.dim() != 2 or t.shape[0] != 3 or t.shape[1] < layout.total_line_elems:
+            raise ValueError(f"{name} must be (3, >= {layout.total_line_elems}), got {tuple(t.shape)}")
+
+    records(charges, "charges")
+    planes(lines, "lines")
+    if steps is not None:
+        tensor(steps, "steps", torch.int32)
+        if steps.numel() < int(layout.n.sum()):
+            raise ValueError("steps buffer smaller than the layout's lines")
+    if ckpt is not None:
+        tensor(ckpt, "ckpt", torch.float32)
+        if ckpt_floats is None or ckpt.numel() < ckpt_floats:
+            raise ValueError(f"checkpoint buffer holds {ckpt.numel()} floats, needs {ckpt_floats}")
+    if grad_lines is not None:
+        planes(grad_lines, "grad_lines")
+    if grad_charges is not None:
+        records(grad_charges, "grad_charges")
+
+
 class BatchPlan:
     """Persistent batched launch over a BatchLayout (optionally hipGraph-replayed)."""
 
@@ -335,29 +382,32 @@ class BatchPlan:
     def run(self, charges, lines, steps=None):
         """Trace every configuration; ``steps`` (optional int32 device tensor of
         sum(n)) receives the step counts, configurations back to back."""
-        torch = _torch()
-        _dev_check(charges, "charges", torch.float32)
-        _dev_check(lines, "lines", torch.float32)
-        if charges.shape[0] < self.layout.total_charges or lines.shape[1] < self.layout.total_line_elems:
-            raise ValueError("packed buffers smaller than the layout")
-        sp = None
-        if steps is not None:
-            _dev_check(steps, "steps", torch.int32)
-            if steps.numel() < int(self.layout.n.sum()):
-                raise ValueError("steps buffer too small")
-            sp = steps.data_ptr()
+        check_batch_buffers(self.layout, charges, lines, steps=steps)
         call("em_batch_run", self._h, charges.data_ptr(), lines[0].data_ptr(), lines[1].data_ptr(),
-             lines[2].data_ptr(), sp, _stream_handle())
+             lines[2].data_ptr(), None if steps is None else steps.data_ptr(), _stream_handle())
 
     def ckpt_size(self, every: int) -> int:
         return int(lib().em_batch_ckpt_floats(self._h, every))
 
     def run_ckpt(self, charges, lines, ckpt, every: int, steps=None):
+        """run() that also stores checkpoints every ``every`` steps into ``ckpt``
+        (at least ckpt_size(every) floats)."""
+        if every < 1:
+            raise ValueError("every must be >= 1")
+        check_batch_buffers(self.layout, charges, lines, steps=steps, ckpt=ckpt,
+                            ckpt_floats=self.ckpt_size(every))
         call("em_batch_run_ckpt", self._h, charges.data_ptr(), lines[0].data_ptr(),
              lines[1].data_ptr(), lines[2].data_ptr(), ckpt.data_ptr(), every,
              None if steps is None else steps.data_ptr(), _stream_handle())
 
     def adjoint(self, charges, lines, ckpt, every: int, grad_lines, grad_charges):
+        """dL/d(charges) into grad_charges (>= sum p, 4) from dL/d(rows) in
+        grad_lines (the packed line layout) and run_ckpt's checkpoints."""
+        if every < 1:
+            raise ValueError("every must be >= 1")
+        check_batch_buffers(self.layout, charges, lines, ckpt=ckpt,
+                            ckpt_floats=self.ckpt_size(every), grad_lines=grad_lines,
+                            grad_charges=grad_charges)
         call("em_batch_adjoint", self._h, charges.data_ptr(), lines[0].data_ptr(),
              lines[1].data_ptr(), lines[2].data_ptr(), ckpt.data_ptr(), every,
              grad_lines[0].data_ptr(), grad_lines[1].data_ptr(), grad_lines[2].data_ptr(),

@@ -49,6 +49,37 @@ class _TraceFn(torch.autograd.Function):
         return grad_c, grad_s.t(), None, None, None, None
 
 
+def check_adjoint_buffers(charges, rows, ckpt, ckpt_every, grad_rows, n=None,
+                          on_device: bool = True) -> None:
+    """Validate a single-configuration adjoint call before any launch: charges
+    (p, 4), rows and grad_rows (3, len, pitch) float32 and contiguous, grad_rows
+    shaped like rows, ckpt_every >= 1 and a checkpoint buffer of at least
+    em_ckpt_floats(n, len, ckpt_every) floats.  Raises TypeError / ValueError."""
+    def tensor(t, name):
+        if on_device:
+            api._dev_check(t, name, torch.float32)
+        elif not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_contiguous():
+            raise TypeError(f"{name} must be a contiguous float32 tensor")
+
+    for t, name in ((charges, "charges"), (rows, "rows"), (ckpt, "ckpt"), (grad_rows, "grad_rows")):
+        tensor(t, name)
+    if charges.dim() != 2 or charges.shape[1] != 4:
+        raise ValueError("charges must be (p, 4)")
+    if rows.dim() != 3 or rows.shape[0] != 3:
+        raise ValueError("rows must be (3, len, pitch)")
+    if tuple(grad_rows.shape) != tuple(rows.shape):
+        raise ValueError(f"grad_rows must be shaped like rows {tuple(rows.shape)}")
+    if ckpt_every < 1:
+        raise ValueError("ckpt_every must be >= 1")
+    _, length, pitch = rows.shape
+    n = pitch if n is None else n
+    if not 0 < n <= pitch:
+        raise ValueError("need 0 < n <= pitch")
+    need = api.ckpt_size(n, length, ckpt_every)
+    if ckpt.numel() < need:
+        raise ValueError(f"checkpoint buffer holds {ckpt.numel()} floats, needs {need}")
+
+
 def trace_lines(charges: torch.Tensor, seeds: torch.Tensor, length: int,
                 rule: StepRule = angle(), ckpt_every: int = 50, launch=None) -> torch.Tensor:
     """Differentiable trace: rows (3, len, n) with row 0 = seeds."""
@@ -57,6 +88,7 @@ def trace_lines(charges: torch.Tensor, seeds: torch.Tensor, length: int,
 
 def adjoint(charges, rows, ckpt, ckpt_every, grad_rows, rule, launch=None, n=None):
     """Direct call of the adjoint (no autograd): returns (grad_charges, grad_seeds[3, n])."""
+    check_adjoint_buffers(charges, rows, ckpt, ckpt_every, grad_rows, n)
     _, length, pitch = rows.shape
     n = pitch if n is None else n
     p = charges.shape[0]
@@ -85,7 +117,8 @@ class _BatchFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_lines):
         charges, lines, ckpt = ctx.saved_tensors
-        grad_c = torch.empty_like(charges)
+        # charges beyond the layout's get no gradient from the adjoint: zero
+        grad_c = torch.zeros_like(charges)
         ctx.plan.adjoint(charges, lines, ckpt, ctx.every, grad_lines.contiguous(), grad_c)
         return grad_c, None, None, None
 

@@ -5,7 +5,12 @@ tile, unroll U = charge records per pipelined group).  The tuner times each
 (S, block, tile) candidate on the caller's own inputs with a reduced row count
 (the per-step cost is what differs between shapes) at the library's default
 unroll, then the fastest of them at U = 8, 4 and 2, and keeps the fastest.
-Every shape gives bit-identical rows; only the time differs.  Results are cached in a JSON file keyed by device,
+The unroll never changes the rows.  The lanes per line S and the tile do:
+S changes how each charge sum is split over lanes, and the tile where the
+compensated partial sums are folded when the charges are re-staged, so rows
+from two shapes agree to float32 summation rounding, not bit for bit.  A
+forward and its adjoint must therefore use the same shape (autograd keeps the
+forward's).  Results are cached in a JSON file keyed by device,
 problem size bucket and step rule (``$EMTRACE_TUNE_CACHE``, default
 ``~/.cache/emtrace/tune.json``); a small table of shapes measured on MI355X
 (``SHIPPED``) answers when the cache has no entry.

@@ -82,6 +82,11 @@ typedef struct em_perf {
                                   formula (CPU_Implement.cpp:347-349)          */
     volatile double progress;  /* 0 at entry, 1.0 when the rows are on the host */
     int32_t lanes_per_line, tile, block, unroll; /* launch shape actually used */
+    volatile double* progress_out; /* optional (may be NULL): every update of
+                                  progress is also stored here while the call
+                                  runs -- e.g. &perfPacket.progress, which the
+                                  reference's monitor thread polls
+                                  (Electromag utils.h:133-156)            */
 } em_perf;
 
 /* ---- library lifetime ----------------------------------------------------*/

@@ -75,6 +75,7 @@ int CalcField_GPU(LinesSoA& fieldLines, ChargeArray& pointCharges, const size_t 
                   "point charges must be {x, y, z, q} records of the line element type");
     const size_t len = n ? fieldLines.GetSize() / n : 0;
     em_perf p = {};
+    p.progress_out = &perfData.progress;  // the caller's monitor sees each row chunk
     const int rc = detail::calc_lines(lines.x, lines.y, lines.z, pointCharges.GetDataPointer(),
                                       pointCharges.GetSize(), n, len, resolution, useCurvature, &p);
     if (rc != EM_OK) return rc;
@@ -105,6 +106,7 @@ int CalcField(LinesAoS& fieldLines, ChargeArray& pointCharges, const size_t n, d
         z[i] = fieldLines[i].z;
     }
     em_perf p = {};
+    p.progress_out = &perfData.progress;
     const int rc = detail::calc_lines(x.data(), y.data(), z.data(), pointCharges.GetDataPointer(),
                                       pointCharges.GetSize(), n, len, resolution, useCurvature, &p);
     if (rc != EM_OK) return rc;

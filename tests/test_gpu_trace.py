@@ -337,3 +337,21 @@ def test_host_path_row_chunks_match_device_path(gpu, rule):
     assert perf["progress"] == 1.0 and perf["kernel_s"] > 0
     assert np.array_equal(steps_h.astype(np.int64), steps_d.cpu().numpy().astype(np.int64))
     assert np.array_equal(rows_h, rows_d.cpu().numpy())
+
+
+def test_drop_in_mirrors_progress(gpu):
+    """em_perf.progress_out: the host-array call stores every progress update
+    there as well (the drop-in points it at perfPacket.progress, which the
+    reference's monitor thread polls); the last value is 1.0."""
+    from emtrace import _lib
+    w = gpu.legacy.preset("default")            # 491 MB of rows: 8 row chunks
+    x, y, z = (np.empty(w.len * w.n, np.float32) for _ in range(3))
+    x[:w.n], y[:w.n], z[:w.n] = w.seeds[:, 0], w.seeds[:, 1], w.seeds[:, 2]
+    seen = ctypes.c_double(-1.0)
+    perf = _lib.em_perf()
+    perf.progress_out = ctypes.pointer(seen)
+    P = lambda a: a.ctypes.data  # noqa: E731
+    rc = gpu.lib().em_calc_lines_f32(P(w.charges), w.p, P(x), P(y), P(z), w.n, w.len, 1.0, 1,
+                                     ctypes.byref(perf))
+    assert rc == 0
+    assert seen.value == 1.0 and perf.progress == 1.0
