@@ -181,12 +181,36 @@ __device__ __forceinline__ void stage(float4* __restrict__ tile, const float4* _
         tile[i] = i < cnt ? src[i] : make_float4(kFar, kFar, kFar, 0.f);
 }
 
-// Butterfly sum over the S lanes of a line group; identical bits in all S lanes.
+// v + (v of the partner lane under a DPP pattern); partners pair up, and IEEE
+// addition commutes, so both lanes of a pair get the same bits.
+template <int CTRL>
+__device__ __forceinline__ float dpp_pair_add(float v)
+{
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the S lanes of a line group (S consecutive lanes, S a power of
+// two <= 64), in registers: lane pairs by DPP within a row of 16 (quad xor 1,
+// quad xor 2, half-row mirror, row mirror), then row pairs and wave halves by
+// the gfx950 lane swaps (v_permlane16_swap / v_permlane32_swap give every
+// lane the even and the odd row of its pair, added in that order).  Every
+// lane of a group ends with the same bits.
 template <int S>
 __device__ __forceinline__ float group_sum(float v)
 {
-#pragma unroll
-    for (int m = S / 2; m >= 1; m /= 2) v += __shfl_xor(v, m, 64);
+    static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "group of 1..64 lanes");
+    if constexpr (S >= 2) v = dpp_pair_add<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    if constexpr (S >= 4) v = dpp_pair_add<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    if constexpr (S >= 8) v = dpp_pair_add<0x141>(v);  // row_half_mirror
+    if constexpr (S >= 16) v = dpp_pair_add<0x140>(v); // row_mirror
+    if constexpr (S >= 32) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+        v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    if constexpr (S >= 64) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+        v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
     return v;
 }
 
