@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extras", default="all",
                     help="comma-separated extra workloads to run (default: all): lines_64k_with_line_ends, "
-                         "lines_8k, batch_512, adjoint_2k, fit_64, fp64_kernel, "
+                         "lines_8k, batch_512, adjoint_2k, fit_64, fit_well_posed, fp64_kernel, "
                          "legacy_default_preset, particle_step")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the FETCH/WRITE_SIZE counters)")
@@ -390,6 +390,32 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
             "note": "iteration = batched forward with checkpoints + dL/dx + batched adjoint + Adam, "
                     "replayed from one captured graph"}
         del fit
+    # 4b. a well-posed inverse problem through the same machinery: 64
+    #     configurations of 2 charges, 256 lines of 60 rows, positions perturbed
+    #     by N(0, 0.02^2), plain rule, 200 iterations with a cosine-decayed rate
+    if on("fit_well_posed"):
+        import math
+        from emtrace.fit import InverseFit
+        wsw = gen.batched_configs(64, (2, 2), (256, 256), 60, 5)
+        truthw = [gen.perturbed(w4.charges, 0.02, seed=100 + i) for i, w4 in enumerate(wsw)]
+        fitw = InverseFit(wsw, truthw, rules.plain(10.0), ckpt_every=20, lr_pos=4e-3, lr_q=2e-3)
+        fitw.plan.run(fitw.theta, fitw.lines)
+        lw0 = fitw.loss()
+        t0 = time.perf_counter()
+        for it in range(200):
+            fitw.set_lr_scale(0.5 * (1.0 + math.cos(math.pi * it / 200)))
+            fitw.step()
+        torch.cuda.synchronize()
+        per_w = (time.perf_counter() - t0) / 200
+        fitw.plan.run(fitw.theta, fitw.lines)
+        res["fit_well_posed"] = {
+            "configs": len(wsw), "charges_per_config": 2, "lines_per_config": 256, "rows": 60,
+            "sigma": 0.02, "rule": "plain", "iterations": 200, "ms_per_iteration": per_w * 1e3,
+            "loss_start": lw0, "loss_end": fitw.loss(), "recovery": fitw.recovery(),
+            "note": "same forward + adjoint + Adam graph as fit_64; learning rate 4e-3 (positions) / "
+                    "2e-3 (magnitudes), cosine-decayed; magnitudes compared modulo each "
+                    "configuration's overall scale (q_rms_scaled), which field lines cannot see"}
+        del fitw
     # 5. FP64 kernel (IEEE binary64, reference op order): lines_64k charges and
     #    lines, 250 rows, plain step
     if on("fp64_kernel"):

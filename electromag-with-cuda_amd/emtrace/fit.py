@@ -79,6 +79,7 @@ class InverseFit:
         self.v = torch.zeros_like(self.theta)
         self.t = torch.zeros((), dtype=torch.float32, device="cuda")
         self.lr = torch.tensor([lr_pos, lr_pos, lr_pos, lr_q], dtype=torch.float32, device="cuda")
+        self.lr0 = self.lr.clone()
         self.b1, self.b2, self.eps = float(betas[0]), float(betas[1]), float(eps)
         self.graph = None
 
@@ -137,6 +138,12 @@ class InverseFit:
         else:
             self.graph.replay()
 
+    def set_lr_scale(self, scale: float):
+        """Learning rates = the initial ones times ``scale`` from the next
+        iteration on (an eager copy into the tensor the graph reads; e.g. a
+        cosine decay over a fixed number of iterations)."""
+        self.lr.copy_(self.lr0 * float(scale))
+
     # ---- reporting (outside the graph) ----
     def loss(self) -> float:
         torch = self.torch
@@ -145,12 +152,30 @@ class InverseFit:
         return float(d.sum().item())
 
     def recovery(self):
-        """RMS position and magnitude errors vs the true charges, now and at start."""
+        """RMS position and magnitude errors vs the true charges, now and at
+        start.  Field lines do not change when all charges of a configuration
+        are scaled by one factor, so the magnitudes are observable only up to
+        it: ``q_rms_scaled`` is the magnitude RMS after each configuration's
+        magnitudes are scaled by the least-squares factor onto the truth."""
         torch = self.torch
+
         def rms(a, b):
             pos = torch.sqrt(((a[:, :3] - b[:, :3]) ** 2).sum(dim=1).mean()).item()
             mag = torch.sqrt(((a[:, 3] - b[:, 3]) ** 2).mean()).item()
             return pos, mag
+
+        def scaled(a, b):
+            sq, cnt = 0.0, 0
+            for i in range(len(self.layout.p)):
+                o, p = int(self.layout.charge_off[i]), int(self.layout.p[i])
+                qa, qb = a[o:o + p, 3].double(), b[o:o + p, 3].double()
+                f = (qa * qb).sum() / (qa * qa).sum()
+                sq += float(((f * qa - qb) ** 2).sum().item())
+                cnt += p
+            return (sq / max(cnt, 1)) ** 0.5
+
         p0, q0 = rms(self.theta0, self.true)
         p1, q1 = rms(self.theta, self.true)
-        return {"pos_rms_start": p0, "pos_rms": p1, "q_rms_start": q0, "q_rms": q1}
+        return {"pos_rms_start": p0, "pos_rms": p1, "q_rms_start": q0, "q_rms": q1,
+                "q_rms_scaled_start": scaled(self.theta0, self.true),
+                "q_rms_scaled": scaled(self.theta, self.true)}

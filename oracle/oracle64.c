@@ -354,9 +354,13 @@ void o64_min_dist_f32(const float* ch, size_t p, const float* pts, size_t m, flo
 
 /* Path distance: for every point j < first[l] of line l in `test` (SoA rows
  * [3][len][n], float32), the distance to the polyline through the points of
- * the same line in `ref` (same layout), searching segments k in [j-W, j+W].
- * Measures how far a traced point lies from the reference path, independent
- * of where along the path it falls.  out: [len][n] (0 beyond first[l]). */
+ * the same line in `ref` (same layout) up to and including point first[l]
+ * (when it exists and is finite: the reference path continues there, so a
+ * test point a little further along the line than the last compared
+ * reference point still finds its segment), searching segments k in
+ * [j-W, j+W].  Measures how far a traced point lies from the reference path,
+ * independent of where along the path it falls.  out: [len][n] (0 beyond
+ * first[l]). */
 void o64_path_dist_f32(const float* test, const float* ref, size_t len, size_t n,
                        const long long* first, long long window, float* out)
 {
@@ -365,11 +369,16 @@ void o64_path_dist_f32(const float* test, const float* ref, size_t len, size_t n
 #pragma omp parallel for schedule(dynamic, 16)
     for (long long l = 0; l < nl; ++l) {
         long long fl = first[l] < (long long)len ? first[l] : (long long)len;
+        /* last reference point of the polyline */
+        long long last = fl - 1;
+        if (fl < (long long)len && fabsf(ref[fl * n + l]) < 1e29f &&
+            fabsf(ref[plane + fl * n + l]) < 1e29f && fabsf(ref[2 * plane + fl * n + l]) < 1e29f)
+            last = fl;
         for (long long j = 0; j < (long long)len; ++j) {
             if (j >= fl) { out[j * n + l] = 0.f; continue; }
             double px = test[j * n + l], py = test[plane + j * n + l], pz = test[2 * plane + j * n + l];
             long long k0 = j - window < 0 ? 0 : j - window;
-            long long k1 = j + window < fl - 1 ? j + window : fl - 1;
+            long long k1 = j + window < last ? j + window : last;
             double best = INFINITY;
             if (k1 <= k0) {
                 double dx = px - ref[j * n + l], dy = py - ref[plane + j * n + l],

@@ -16,9 +16,18 @@ import torch
 
 from emtrace import gen, rules
 from emtrace.fit import InverseFit
-from helpers import rows64_to_soa
+from helpers import fp64_sensitivity, rows64_to_soa
 
 pytestmark = pytest.mark.gpu
+# The FP64 adjoint of the FP64 trajectory is compared on lines that follow it
+# (same step count, every point within FOLLOW_TOL) and are well conditioned:
+# their FP64 path moves by at most SENS_TOL under float32-sized rounding
+# (helpers.fp64_sensitivity).  On lines that amplify rounding more, the
+# float32 and FP64 paths are linearised on measurably different trajectories.
+# Measured on the 4-configuration subset: 0.9e-3 - 4.0e-3 with SENS_TOL = 1e-4
+# (73-78 % of the lines), up to 8e-2 when lines up to 1e-3 are admitted.
+FOLLOW_TOL = 1e-3
+SENS_TOL = 1e-4
 
 
 def _problem(count, p_range, n_range, length, sigma, seed):
@@ -80,16 +89,22 @@ def test_fit_64_subset_gradients_vs_fp64(gpu, orc):
         # own FP64 trajectory, on the lines that follow it
         r64, s64 = orc.trace64(w.charges, seeds, w.len, rules.angle(), with_steps=True)
         err = np.abs(rows.astype(np.float64) - rows64_to_soa(r64)).max(axis=(0, 1))
-        ok = (st == s64.astype(np.int64)) & (err <= 1e-3)
+        follow = (st == s64.astype(np.int64)) & (err <= FOLLOW_TOL)
+        first = np.full(w.n, w.len)
+        sens = np.zeros(w.n)
+        idx = np.nonzero(follow)[0]
+        sens[idx] = fp64_sensitivity(w.charges, seeds, w.len, rules.angle(), first, idx)
+        ok = follow & (sens <= SENS_TOL)
         Gm = G64.copy()
         Gm[:, ~ok, :] = 0.0
         g64, _ = orc.adjoint64(w.charges, seeds, w.len, rules.angle(), Gm)
         Gmf = np.ascontiguousarray(Gm.transpose(2, 0, 1)).astype(np.float32)
         rep.append((w.p, w.n, float(r_path), float(ok.mean()), g64, Gmf, i))
         print(f"[fit_64 subset] cfg {i}: p={w.p} n={w.n} rel(gpu, FP64 along gpu path) {r_path:.2e}; "
-              f"lines following FP64 {ok.mean():.3f}", flush=True)
+              f"lines following FP64 {follow.mean():.3f}, and well conditioned {ok.mean():.3f}",
+              flush=True)
         assert r_path < 1e-3
-        assert ok.mean() > 0.8
+        assert follow.mean() > 0.8 and ok.mean() > 0.6
     # the masked gradients (agreeing lines only) through the batched GPU adjoint
     for (p, n, _, _, g64, Gmf, i) in rep:
         fit.layout.rows_view(fit.G, i).copy_(torch.from_numpy(Gmf).cuda())
@@ -100,7 +115,7 @@ def test_fit_64_subset_gradients_vs_fp64(gpu, orc):
         o = int(fit.layout.charge_off[i])
         r = np.linalg.norm(grad[o:o + p] - g64) / np.linalg.norm(g64)
         print(f"[fit_64 subset] cfg {i}: rel(gpu, FP64 adjoint of FP64 trajectory) {r:.2e}", flush=True)
-        assert r < 1e-1
+        assert r < 1e-2
 
 
 def test_scaled_gradient_equals_weighted_dldx(gpu):
@@ -150,3 +165,25 @@ def test_lines_ordered_by_step_count(gpu):
         assert (np.diff(st.astype(np.int64)) <= 0).all(), i  # longest first
         seeds = fit.layout.rows_view(fit.lines, i)[:, 0, :].t().cpu().numpy()
         assert np.array_equal(seeds, w.seeds[fit.order[i]]), i
+
+
+def test_well_posed_fit_recovers_the_charges(gpu):
+    """A well-posed inverse problem -- 64 configurations of 2 charges, 256
+    lines of 60 rows, positions perturbed by N(0, 0.02^2) (a fifth of the seed
+    radius), the plain rule (row s sits at arc length s / resolution) -- is
+    recovered by the same machinery: 200 captured iterations with a cosine-
+    decayed learning rate bring the position RMS error down by more than 2x
+    and the magnitude error (modulo each configuration's overall scale, which
+    the lines cannot see) by more than 1.5x."""
+    import math
+    ws = gen.batched_configs(64, (2, 2), (256, 256), 60, 5)
+    truth = [gen.perturbed(w.charges, 0.02, seed=100 + i) for i, w in enumerate(ws)]
+    fit = InverseFit(ws, truth, rules.plain(10.0), ckpt_every=20, lr_pos=4e-3, lr_q=2e-3)
+    for it in range(200):
+        fit.set_lr_scale(0.5 * (1.0 + math.cos(math.pi * it / 200)))
+        fit.step()
+    torch.cuda.synchronize()
+    r = fit.recovery()
+    print(f"[fit well-posed] {r}", flush=True)
+    assert r["pos_rms"] < 0.5 * r["pos_rms_start"]
+    assert r["q_rms_scaled"] < r["q_rms_scaled_start"] / 1.5

@@ -370,8 +370,9 @@ def test_min_dist_matches_brute_force(orc):
 
 def test_path_dist_matches_brute_force(orc):
     """With a window covering the whole line, o64_path_dist_f32 is the
-    distance to the nearest segment of the reference polyline before first[l];
-    0 at and after first[l]."""
+    distance to the nearest segment of the reference polyline through points
+    0 .. first[l] (the point where the comparison stops still bounds the last
+    segment); 0 at and after first[l]."""
     rng = np.random.default_rng(13)
     length, n = 12, 5
     ref = np.cumsum(rng.normal(size=(3, length, n)), axis=1).astype(np.float32)
@@ -384,9 +385,10 @@ def test_path_dist_matches_brute_force(orc):
                 assert got[j, l] == 0.0
                 continue
             p = test[:, j, l].astype(np.float64)
+            last = min(first[l], length - 1)
             want = min(_seg_dist_cross(p, ref[:, k, l].astype(np.float64),
                                        ref[:, k + 1, l].astype(np.float64))
-                       for k in range(first[l] - 1))
+                       for k in range(last))
             assert abs(got[j, l] - want) <= 1e-6 * max(1.0, want), (j, l, got[j, l], want)
 
 
@@ -434,9 +436,13 @@ def test_path_dist_geometry(orc):
     assert narrow.min() > 1.5      # the 4-segment window sees only points near angle j
 
 
-def test_path_dist_pointwise_fallback(orc):
-    """Where the window holds no segment (window 0, or a line with one point
-    before first[l]) the distance is pointwise, to the point of the same index."""
+def test_path_dist_pointwise_fallback_and_prefix_end(orc):
+    """Where the window holds no segment (window 0, or a one-row line) the
+    distance is pointwise, to the point of the same index.  A line compared up
+    to first[l] = 1 is measured against its first segment (points 0 and 1);
+    a point just past the end of the compared prefix along the path is at
+    distance 0 from that segment, not at its along-line offset; a non-finite
+    point at first[l] (a reference path that blew up there) is not used."""
     rng = np.random.default_rng(14)
     ref = rng.normal(size=(3, 9, 3)).astype(np.float32)
     test = rng.normal(size=(3, 9, 3)).astype(np.float32)
@@ -444,8 +450,23 @@ def test_path_dist_pointwise_fallback(orc):
     d0 = orc.path_dist(test, ref, np.array([9, 9, 9]), 0)
     np.testing.assert_allclose(d0, pw, rtol=1e-6)
     d1 = orc.path_dist(test, ref, np.array([1, 9, 1]), 3)
-    np.testing.assert_allclose(d1[0, [0, 2]], pw[0, [0, 2]], rtol=1e-6)
+    for l in (0, 2):
+        want = _seg_dist_cross(test[:, 0, l].astype(np.float64), ref[:, 0, l].astype(np.float64),
+                               ref[:, 1, l].astype(np.float64))
+        assert abs(d1[0, l] - want) <= 1e-6 * max(1.0, want)
     assert (d1[1:, [0, 2]] == 0).all()
+    one = orc.path_dist(test[:, :1], ref[:, :1], np.array([1, 1, 1]), 3)
+    np.testing.assert_allclose(one[0], pw[0], rtol=1e-6)
+    # straight reference along x with steps 0.3; compared up to first = 5; the
+    # test's point 4 lies 0.01 further along the line than the reference's
+    line = np.zeros((3, 8, 1), np.float32)
+    line[0, :, 0] = 0.3 * np.arange(8)
+    t = line.copy()
+    t[0, 4, 0] += 0.01
+    assert orc.path_dist(t, line, np.array([5]), 4)[4, 0] < 1e-7
+    bad = line.copy()
+    bad[:, 5, 0] = 1e30                     # the reference is non-finite at first[l]
+    np.testing.assert_allclose(orc.path_dist(t, bad, np.array([5]), 4)[4, 0], 0.01, rtol=1e-4)
 
 
 def test_trace64_plain_closed_form(orc):
