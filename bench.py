@@ -460,6 +460,18 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                 raise SystemExit(f"drop-in call failed: {rc}")
             del hx, hy, hz
         wall_d, perf_d = min(calls, key=lambda c: c[0])
+        # the same call on doubles (CalcField_GPU<double> -> em_calc_lines_f64)
+        calls64 = []
+        for _ in range(2):
+            hx, hy, hz = (np.empty(wl.len * wl.n, np.float64) for _ in range(3))
+            hx[:wl.n], hy[:wl.n], hz[:wl.n] = wl.seeds[:, 0], wl.seeds[:, 1], wl.seeds[:, 2]
+            t0 = time.perf_counter()
+            rc, perf = emtrace.calc_lines64(wl.charges.astype(np.float64), hx, hy, hz, wl.n, 1.0, True)
+            calls64.append((time.perf_counter() - t0, perf))
+            if rc != 0:
+                raise SystemExit(f"FP64 drop-in call failed: {rc}")
+            del hx, hy, hz
+        wall_64, perf_64 = min(calls64, key=lambda c: c[0])
         res["legacy_default_preset"] = {
             "interactions_per_s": float(wl.interactions) / min(kl), "kernel_ms": min(kl) * 1e3,
             "charges": wl.p, "lines": wl.n, "rows": wl.len,
@@ -470,7 +482,14 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                 "note": "em_calc_lines_f32 on fresh numpy arrays (best of 2 calls); the reference "
                         "app's CalcField call. Rows are traced in row chunks; each chunk's copy "
                         "back into the pageable arrays (d2h_s sums the transfers) overlaps the "
-                        "next chunk's kernel"}}
+                        "next chunk's kernel"},
+            "dropin_host_arrays_f64": {
+                "call_s": wall_64, "h2d_s": perf_64["h2d_s"], "kernel_s": perf_64["kernel_s"],
+                "d2h_s": perf_64["d2h_s"],
+                "d2h_GB_per_s": perf_64["d2h_bytes"] / perf_64["d2h_s"] / 1e9,
+                "interactions_per_s": perf_64["interactions_per_s"],
+                "note": "em_calc_lines_f64 (the IEEE binary64 kernel) on fresh float64 arrays, best "
+                        "of 2 calls; cached device buffers, row chunks as the FP32 call"}}
         del rl, cl, sl
     # 7. particle step (Particle_System.cpp:108-190): 65,536 particles among the
     #    16,384 lines_64k charges, one step with the written-back state

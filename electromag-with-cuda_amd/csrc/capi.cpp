@@ -240,32 +240,17 @@ static int check_sizes(size_t p, size_t n, size_t len, const em_step* st)
 
 // ---------------------------------------------------------------------------
 // device buffer cache for the host-array forms (one set per device)
-constexpr size_t kMaxChunks = 8;                // row chunks of a host-array trace
-constexpr size_t kChunkBytes = 48ull << 20;     // output bytes per row chunk
+constexpr size_t kChunkBytes = 48ull << 20;  // output bytes per row chunk
 
-struct HostPathBuffers {
-    void* charges = nullptr;
-    size_t charges_bytes = 0;
-    float* lines = nullptr;  // 3 * len * n floats
-    size_t lines_bytes = 0;
-    void* steps = nullptr;   // n step counts
-    size_t steps_bytes = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t copy = nullptr;  // copies back of finished row chunks, beside the kernels
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t chunk_ev[kMaxChunks] = {};
-};
+static std::mutex g_hp_mu;  // held by a host-array call from set-up to the last copy
 static HostPathBuffers g_hp[64];
 
-// Row chunks of a host-array trace: the copy back of chunk k runs while the
-// kernel of chunk k+1 traces.  Possible for the plain and ref rules, whose
-// state is the point alone (a chunk continues from the previous chunk's last
-// row); one chunk per 48 MiB of output, at most 8.  Only the last chunk's
-// copy is exposed: on the reference's default problem (491 MB of rows) eight
-// chunks take 20.1-20.4 ms per call against 21.7 ms with four (sixteen: within 2 %).
-static uint32_t host_chunks(const StepP& sp, bool ends, size_t len, size_t out_bytes)
+// Only the last chunk's copy is exposed: on the reference's default problem
+// (491 MB of rows) eight chunks take 20.1-20.4 ms per call against 21.7 ms
+// with four (sixteen: within 2 %).
+uint32_t host_chunks(bool chunkable, size_t len, size_t out_bytes)
 {
-    if (ends || (sp.rule != kRulePlain && sp.rule != kRuleRef) || len < 16) return 1;
+    if (!chunkable || len < 16) return 1;
     return (uint32_t)std::min<size_t>(kMaxChunks, std::max<size_t>(1, out_bytes / kChunkBytes));
 }
 
@@ -277,6 +262,49 @@ static int ensure(void** ptr, size_t* have, size_t need)
     *have = 0;
     EM_HIP(hipMalloc(ptr, need));
     *have = need;
+    return EM_OK;
+}
+
+int host_path_acquire(int device, size_t charge_bytes, size_t line_bytes, size_t step_bytes,
+                      std::unique_lock<std::mutex>& lk, HostPathBuffers** out)
+{
+    if (device < 0 || device >= 64) EM_FAIL(EM_ERR_NO_DEVICE, "device index %d out of range", device);
+    lk = std::unique_lock<std::mutex>(g_hp_mu);
+    HostPathBuffers& b = g_hp[device];
+    if (!b.stream) {
+        EM_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        EM_HIP(hipStreamCreateWithFlags(&b.copy, hipStreamNonBlocking));
+        for (auto& e : b.ev) EM_HIP(hipEventCreate(&e));
+        for (auto& e : b.chunk_ev) EM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    EM_TRY(ensure(&b.charges, &b.charges_bytes, std::max<size_t>(charge_bytes, 16)));
+    EM_TRY(ensure(&b.lines, &b.lines_bytes, line_bytes));
+    if (step_bytes) EM_TRY(ensure(&b.steps, &b.steps_bytes, step_bytes));
+    *out = &b;
+    return EM_OK;
+}
+
+int host_path_release_all()
+{
+    std::lock_guard<std::mutex> lk(g_hp_mu);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return EM_OK;
+    for (int i = 0; i < 64; ++i) {
+        HostPathBuffers& b = g_hp[i];
+        if (!b.stream && !b.charges && !b.lines) continue;
+        EM_HIP(hipSetDevice(i));
+        if (b.charges) EM_HIP(hipFree(b.charges));
+        if (b.lines) EM_HIP(hipFree(b.lines));
+        if (b.steps) EM_HIP(hipFree(b.steps));
+        for (auto& e : b.ev)
+            if (e) EM_HIP(hipEventDestroy(e));
+        for (auto& e : b.chunk_ev)
+            if (e) EM_HIP(hipEventDestroy(e));
+        if (b.stream) EM_HIP(hipStreamDestroy(b.stream));
+        if (b.copy) EM_HIP(hipStreamDestroy(b.copy));
+        b = HostPathBuffers();
+    }
+    EM_HIP(hipSetDevice(cur));
     return EM_OK;
 }
 
@@ -315,20 +343,13 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     const bool ends = st->stop != 0 && p >= kEndsMinCharges;
     EM_TRY(choose_shape(dev, p, n, launch, sh, st->stop != 0, ends));
 
-    std::lock_guard<std::mutex> lk(g_mu);
-    HostPathBuffers& b = g_hp[dev.device];
-    if (!b.stream) {
-        EM_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
-        EM_HIP(hipStreamCreateWithFlags(&b.copy, hipStreamNonBlocking));
-        for (auto& e : b.ev) EM_HIP(hipEventCreate(&e));
-        for (auto& e : b.chunk_ev) EM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
-    const size_t cb = std::max<size_t>(p, 1) * sizeof(float4);
     const size_t row = n * sizeof(float);
-    EM_TRY(ensure(&b.charges, &b.charges_bytes, cb));
-    EM_TRY(ensure(reinterpret_cast<void**>(&b.lines), &b.lines_bytes, 3 * len * row));
-    if (steps) EM_TRY(ensure(&b.steps, &b.steps_bytes, n * sizeof(uint32_t)));
-    float* dx = b.lines;
+    std::unique_lock<std::mutex> lk;
+    HostPathBuffers* hp = nullptr;
+    EM_TRY(host_path_acquire(dev.device, p * sizeof(float4), 3 * len * row,
+                             steps ? n * sizeof(uint32_t) : 0, lk, &hp));
+    HostPathBuffers& b = *hp;
+    float* dx = static_cast<float*>(b.lines);
     float* dy = dx + len * n;
     float* dz = dy + len * n;
 
@@ -348,7 +369,8 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     const StepP sp = to_stepp(*st);
     uint32_t* dsteps = steps ? static_cast<uint32_t*>(b.steps) : nullptr;
     const size_t out_bytes = (len - 1) * row;
-    const uint32_t chunks = host_chunks(sp, ends, len, 3 * out_bytes);
+    const uint32_t chunks =
+        host_chunks(!ends && (sp.rule == kRulePlain || sp.rule == kRuleRef), len, 3 * out_bytes);
     // chunk c covers rows bound(c) .. bound(c+1)-1
     auto bound = [&](uint32_t c) { return (uint32_t)(1 + (uint64_t)(len - 1) * c / chunks); };
     if (ends) {
@@ -441,25 +463,7 @@ int em_init(int device)
 int em_shutdown(void)
 {
     clear_error();
-    std::lock_guard<std::mutex> lk(g_mu);
-    int cur = 0;
-    if (hipGetDevice(&cur) != hipSuccess) return EM_OK;
-    for (int i = 0; i < 64; ++i) {
-        HostPathBuffers& b = g_hp[i];
-        if (!b.stream && !b.charges && !b.lines) continue;
-        EM_HIP(hipSetDevice(i));
-        if (b.charges) EM_HIP(hipFree(b.charges));
-        if (b.lines) EM_HIP(hipFree(b.lines));
-        if (b.steps) EM_HIP(hipFree(b.steps));
-        for (auto& e : b.ev)
-            if (e) EM_HIP(hipEventDestroy(e));
-        for (auto& e : b.chunk_ev)
-            if (e) EM_HIP(hipEventDestroy(e));
-        if (b.stream) EM_HIP(hipStreamDestroy(b.stream));
-        if (b.copy) EM_HIP(hipStreamDestroy(b.copy));
-        b = HostPathBuffers();
-    }
-    EM_HIP(hipSetDevice(cur));
+    EM_TRY(host_path_release_all());
     EM_TRY(staging_release_all());
     EM_TRY(ends_release_all());
     return adj_release_all();

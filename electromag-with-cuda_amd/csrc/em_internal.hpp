@@ -7,6 +7,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -140,6 +141,33 @@ int noalloc_scratch(int device, char** out);
 int launch_field(const Shape& sh, const float4* charges, uint32_t p, const float* px,
                  const float* py, const float* pz, uint64_t m, float scale, float* ex, float* ey,
                  float* ez, hipStream_t stream);
+
+// Device buffers, two streams and events kept per device for the host-array
+// entry points (the FP32 and FP64 forms share one set): charges, the lines'
+// rows and step counts, grown on demand and kept until em_shutdown.
+constexpr size_t kMaxChunks = 8;  // row chunks of a host-array trace
+struct HostPathBuffers {
+    void* charges = nullptr;
+    size_t charges_bytes = 0;
+    void* lines = nullptr;  // 3 * len * n elements
+    size_t lines_bytes = 0;
+    void* steps = nullptr;  // n step counts
+    size_t steps_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t copy = nullptr;  // copies back of finished row chunks, beside the kernels
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t chunk_ev[kMaxChunks] = {};
+};
+// Takes `lk` on the set of `device` (current) for the caller's whole call,
+// creates its streams and events on first use and grows its buffers to at
+// least the given sizes.
+int host_path_acquire(int device, size_t charge_bytes, size_t line_bytes, size_t step_bytes,
+                      std::unique_lock<std::mutex>& lk, HostPathBuffers** out);
+int host_path_release_all();
+// Row chunks of a host-array trace whose rule carries no state besides the
+// point and the seed (chunk k's copy back overlaps chunk k+1's kernel): one
+// per 48 MiB of output, at most kMaxChunks; 1 when `chunkable` is false.
+uint32_t host_chunks(bool chunkable, size_t len, size_t out_bytes);
 
 // device -> pageable host copies through pinned staging buffers (staging.cpp);
 // issued on `stream` after the work that produces `src`, returns when every

@@ -12,7 +12,9 @@
 // row equals the FP64 restatement of the reference algorithm bit for bit.
 //
 // Layout: one lane per line (a lane owns its line's whole charge sum, which
-// fixes the summation order); 256 lines per workgroup; charges staged through
+// fixes the summation order), or, for problems with too few lines to give
+// every SIMD a wave, a group of 2 or 4 lanes per line that splits the terms
+// and adds them in the same order; 256 lanes per workgroup; charges staged through
 // LDS as 32-byte records, all of them once when they fit a tile, else tile by
 // tile each step.  The charge loop is unrolled so the independent per-charge
 // terms (a square root and three divisions each) overlap while the additions
@@ -91,6 +93,75 @@ __device__ __forceinline__ void sum_range(const double4* __restrict__ tile, uint
     }
 }
 
+// Lines with fewer than one lane each per SIMD's wave slot leave SIMDs idle:
+// such problems give each line an S-lane group (S = 2 or 4, within one quad of
+// lanes).  Lane k computes the terms of charges i+k, i+S+k, ... and every lane
+// of the group adds all of the group's terms in charge order, lane j's term
+// read by a quad permutation; the group then holds identical sums, added in
+// the order of the one-lane loop.
+template <int S, int J>
+__device__ __forceinline__ double from_lane(double v)
+{
+    static_assert(S == 2 || S == 4, "groups of 2 or 4 lanes");
+    // quad_perm selects: lane J of {0,1} and of {2,3}, or lane J of {0..3}
+    constexpr int ctrl = S == 4 ? J * 0x55 : (J | (J << 2) | ((2 + J) << 4) | ((2 + J) << 6));
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// s += the terms of lanes 0 .. m-1 of the group, in lane order (m <= S)
+template <int S>
+__device__ __forceinline__ void add_in_order(double& s, double v, uint32_t m)
+{
+    if constexpr (S == 1) {
+        s += v;
+    } else {
+        s += from_lane<S, 0>(v);
+        if (m > 1) s += from_lane<S, 1>(v);
+        if constexpr (S == 4) {
+            if (m > 2) s += from_lane<S, 2>(v);
+            if (m > 3) s += from_lane<S, 3>(v);
+        }
+    }
+}
+
+template <int RULE, int S>
+__device__ __forceinline__ void sum_range_group(const double4* __restrict__ tile, uint32_t count,
+                                                uint32_t k, double px, double py, double pz,
+                                                double& sx, double& sy, double& sz)
+{
+    if constexpr (S == 1) {
+        sum_range<RULE>(tile, count, px, py, pz, sx, sy, sz);
+    } else {
+        constexpr uint32_t G = S * kUnroll64;  // charges per pass
+        uint32_t i = 0;
+        for (; i + G <= count; i += G) {
+            double e[kUnroll64][3];
+#pragma unroll
+            for (int u = 0; u < kUnroll64; ++u)
+                term64<RULE>(tile[i + u * S + k], px, py, pz, e[u][0], e[u][1], e[u][2]);
+#pragma unroll
+            for (int u = 0; u < kUnroll64; ++u) {
+                add_in_order<S>(sx, e[u][0], S);
+                add_in_order<S>(sy, e[u][1], S);
+                add_in_order<S>(sz, e[u][2], S);
+            }
+        }
+        for (; i < count; i += S) {
+            // the group's last charges: m = fewer than S terms may remain
+            // (uniform over the workgroup: every lane sees the same count)
+            const uint32_t m = min((uint32_t)S, count - i);
+            double ex = 0.0, ey = 0.0, ez = 0.0;
+            if (k < m) term64<RULE>(tile[i + k], px, py, pz, ex, ey, ez);
+            add_in_order<S>(sx, ex, m);
+            add_in_order<S>(sy, ey, m);
+            add_in_order<S>(sz, ez, m);
+        }
+    }
+}
+
 struct Line64 {
     double px, py, pz;  // current point
     double Px, Py, Pz;  // seed (the reference's prevAccum)
@@ -148,55 +219,66 @@ __device__ __forceinline__ void step64(const StepP64& sp, uint32_t s, double ex,
     st.steps = s;
 }
 
-template <int RULE>
+template <int RULE, int S>
 __global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict__ charges,
                                                       uint32_t p, double* __restrict__ x,
                                                       double* __restrict__ y,
                                                       double* __restrict__ z, uint32_t n,
                                                       uint32_t len, uint64_t pitch, StepP64 sp,
-                                                      uint32_t tile, uint32_t* steps_out)
+                                                      uint32_t tile, uint32_t* steps_out,
+                                                      uint32_t row0)
 {
     extern __shared__ double4 lds64[];
-    const uint32_t line = blockIdx.x * kBlock64 + threadIdx.x;
+    const uint32_t k = threadIdx.x % S;
+    const uint32_t line = blockIdx.x * (kBlock64 / S) + threadIdx.x / S;
     const bool valid = line < n;
+    const bool writer = valid && k == 0;
     Line64 st{};
     if (valid) {
-        st.px = st.Px = x[line];
-        st.py = st.Py = y[line];
-        st.pz = st.Pz = z[line];
+        // rows row0 .. len-1; row0 > 1 (plain and ref rules) continues from
+        // row row0-1, the seed staying row 0
+        const uint64_t from = (uint64_t)(row0 - 1) * pitch + line;
+        st.px = x[from];
+        st.py = y[from];
+        st.pz = z[from];
+        st.Px = x[line];
+        st.Py = y[line];
+        st.Pz = z[line];
     }
     st.h = sp.h0;
+    st.steps = row0 - 1;
     st.alive = valid;
     const bool single = p <= tile;
     if (single) {
         for (uint32_t i = threadIdx.x; i < p; i += kBlock64) lds64[i] = charges[i];
         __syncthreads();
     }
-    for (uint32_t s = 1; s < len; ++s) {
+    for (uint32_t s = row0; s < len; ++s) {
         // the workgroup skips the field once none of its lines is alive
         if (__syncthreads_or(st.alive)) {
             double ex = 0.0, ey = 0.0, ez = 0.0;
             if (single) {
-                if (st.alive) sum_range<RULE>(lds64, p, st.px, st.py, st.pz, ex, ey, ez);
+                if (st.alive) sum_range_group<RULE, S>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez);
             } else {
                 for (uint32_t t0 = 0; t0 < p; t0 += tile) {
                     const uint32_t cnt = min(tile, p - t0);
                     __syncthreads();
                     for (uint32_t i = threadIdx.x; i < cnt; i += kBlock64) lds64[i] = charges[t0 + i];
                     __syncthreads();
-                    if (st.alive) sum_range<RULE>(lds64, cnt, st.px, st.py, st.pz, ex, ey, ez);
+                    if (st.alive)
+                        sum_range_group<RULE, S>(lds64, cnt, k, st.px, st.py, st.pz, ex, ey, ez);
                 }
             }
             if (st.alive) step64<RULE>(sp, s, ex, ey, ez, st);
         }
-        if (valid) {
+        if (writer) {
             const uint64_t at = (uint64_t)s * pitch + line;
             x[at] = st.px;
             y[at] = st.py;
             z[at] = st.pz;
         }
     }
-    if (valid && steps_out) steps_out[line] = st.steps;
+    if (writer && steps_out) steps_out[line] = st.steps;
 }
 
 int validate64(size_t p, size_t n, size_t len, const StepP64& sp)
@@ -234,25 +316,62 @@ StepP64 from_step(const em_step& st)
     return sp;
 }
 
-int launch64(const double* charges, uint32_t p, double* x, double* y, double* z, uint32_t n,
-             uint32_t len, uint64_t pitch, const StepP64& sp, uint32_t* steps, hipStream_t stream)
+// Rows row0 .. len-1 (row0 > 1: plain and ref rules only).
+// Lanes per line.  One lane per line gives every SIMD a wave (256 CUs x 4
+// SIMDs x 64 lanes = 65,536 lanes) only from 65,536 lines on: fewer lines take
+// S = 2 lanes each, and at most half that many, S = 4.
+uint32_t lanes64(const DeviceInfo& dev, uint32_t n)
 {
+    const uint64_t simd_lanes = (uint64_t)dev.cus * 4 * 64;
+    if ((uint64_t)n * 2 <= simd_lanes) return 4;
+    if ((uint64_t)n < simd_lanes) return 2;
+    return 1;
+}
+
+template <int RULE, int S>
+void launch64_s(dim3 grid, size_t lds, hipStream_t stream, const double4* q, uint32_t p, double* x,
+                double* y, double* z, uint32_t n, uint32_t len, uint64_t pitch, const StepP64& sp,
+                uint32_t tile, uint32_t* steps, uint32_t row0)
+{
+    hipLaunchKernelGGL((k_trace64<RULE, S>), grid, dim3(kBlock64), lds, stream, q, p, x, y, z, n,
+                       len, pitch, sp, tile, steps, row0);
+}
+
+template <int RULE>
+void launch64_rule(uint32_t S, size_t lds, hipStream_t stream, const double4* q, uint32_t p,
+                   double* x, double* y, double* z, uint32_t n, uint32_t len, uint64_t pitch,
+                   const StepP64& sp, uint32_t tile, uint32_t* steps, uint32_t row0)
+{
+    const uint32_t per = kBlock64 / S;
+    const dim3 grid((n + per - 1) / per);
+    if (S == 4)
+        launch64_s<RULE, 4>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+    else if (S == 2)
+        launch64_s<RULE, 2>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+    else
+        launch64_s<RULE, 1>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+}
+
+// Rows row0 .. len-1 (row0 > 1: plain and ref rules only).
+int launch64(const DeviceInfo& dev, const double* charges, uint32_t p, double* x, double* y,
+             double* z, uint32_t n, uint32_t len, uint64_t pitch, const StepP64& sp,
+             uint32_t* steps, hipStream_t stream, uint32_t row0 = 1)
+{
+    if (row0 < 1 || (row0 > 1 && sp.rule == kRuleAngle))
+        EM_FAIL(EM_ERR_INVALID, "row0 %u: the angle rule traces from row 1", row0);
     const uint32_t tile = p <= kTile64 ? std::max<uint32_t>(p, 1) : kTile64 / 2;
     const size_t lds = (size_t)tile * sizeof(double4);
-    const dim3 grid((n + kBlock64 - 1) / kBlock64), block(kBlock64);
+    const uint32_t S = lanes64(dev, n);
     const double4* q = reinterpret_cast<const double4*>(charges);
     switch (sp.rule) {
     case kRulePlain:
-        hipLaunchKernelGGL(k_trace64<kRulePlain>, grid, block, lds, stream, q, p, x, y, z, n, len,
-                           pitch, sp, tile, steps);
+        launch64_rule<kRulePlain>(S, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
         break;
     case kRuleRef:
-        hipLaunchKernelGGL(k_trace64<kRuleRef>, grid, block, lds, stream, q, p, x, y, z, n, len,
-                           pitch, sp, tile, steps);
+        launch64_rule<kRuleRef>(S, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
         break;
     default:
-        hipLaunchKernelGGL(k_trace64<kRuleAngle>, grid, block, lds, stream, q, p, x, y, z, n, len,
-                           pitch, sp, tile, steps);
+        launch64_rule<kRuleAngle>(S, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
         break;
     }
     EM_HIP(hipGetLastError());
@@ -265,7 +384,9 @@ double now64()
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Host arrays in, host arrays out; device buffers live for the call only.
+// Host arrays in, host arrays out, through the per-device buffers, streams and
+// events the FP32 host-array forms keep (host_path_acquire).  The plain and ref
+// rules trace in row chunks whose copies back overlap the next chunk's kernel.
 int host_trace64(const double* charges, size_t p, double* x, double* y, double* z, size_t n,
                  size_t len, const StepP64& sp, em_perf* perf, uint32_t* steps)
 {
@@ -280,53 +401,52 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
 
-    struct Res {  // released on every return path
-        void* q = nullptr;
-        void* lines = nullptr;
-        void* st = nullptr;
-        hipStream_t stream = nullptr;
-        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-        ~Res()
-        {
-            if (stream) (void)hipStreamSynchronize(stream);
-            if (q) (void)hipFree(q);
-            if (lines) (void)hipFree(lines);
-            if (st) (void)hipFree(st);
-            for (auto& e : ev)
-                if (e) (void)hipEventDestroy(e);
-            if (stream) (void)hipStreamDestroy(stream);
-        }
-    } r;
     const size_t row = n * sizeof(double);
-    EM_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
-    for (auto& e : r.ev) EM_HIP(hipEventCreate(&e));
-    EM_HIP(hipMalloc(&r.q, std::max<size_t>(p, 1) * sizeof(double4)));
-    EM_HIP(hipMalloc(&r.lines, 3 * len * row));
-    if (steps) EM_HIP(hipMalloc(&r.st, n * sizeof(uint32_t)));
-    double* dx = static_cast<double*>(r.lines);
+    std::unique_lock<std::mutex> lk;
+    HostPathBuffers* hp = nullptr;
+    EM_TRY(host_path_acquire(dev.device, p * sizeof(double4), 3 * len * row,
+                             steps ? n * sizeof(uint32_t) : 0, lk, &hp));
+    HostPathBuffers& b = *hp;
+    double* dx = static_cast<double*>(b.lines);
     double* dy = dx + len * n;
     double* dz = dy + len * n;
+    uint32_t* dsteps = steps ? static_cast<uint32_t*>(b.steps) : nullptr;
 
-    EM_HIP(hipEventRecord(r.ev[0], r.stream));
+    EM_HIP(hipEventRecord(b.ev[0], b.stream));
     const CopySeg in[4] = {{dx, x, row}, {dy, y, row}, {dz, z, row},
-                           {r.q, charges, p * sizeof(double4)}};
-    EM_TRY(h2d_staged(dev.device, in, p ? 4 : 3, r.stream));
-    EM_HIP(hipEventRecord(r.ev[1], r.stream));
-    EM_TRY(launch64(static_cast<const double*>(r.q), (uint32_t)p, dx, dy, dz, (uint32_t)n,
-                    (uint32_t)len, n, sp, static_cast<uint32_t*>(r.st), r.stream));
-    EM_HIP(hipEventRecord(r.ev[2], r.stream));
+                           {b.charges, charges, p * sizeof(double4)}};
+    EM_TRY(h2d_staged(dev.device, in, p ? 4 : 3, b.stream));
+    EM_HIP(hipEventRecord(b.ev[1], b.stream));
     const size_t out_bytes = (len - 1) * row;
-    const CopySeg segs[4] = {{x + n, dx + n, out_bytes}, {y + n, dy + n, out_bytes},
-                             {z + n, dz + n, out_bytes}, {steps, r.st, n * sizeof(uint32_t)}};
-    prefault_pages(segs, 3);  // while the GPU traces
-    EM_HIP(hipEventSynchronize(r.ev[2]));
-    const double td = now64();
-    EM_TRY(d2h_staged(dev.device, segs, steps ? 4 : 3, r.stream));
-    const double d2h = now64() - td;
+    const uint32_t chunks =
+        host_chunks(sp.rule == kRulePlain || sp.rule == kRuleRef, len, 3 * out_bytes);
+    // chunk c covers rows bound(c) .. bound(c+1)-1
+    auto bound = [&](uint32_t c) { return (uint32_t)(1 + (uint64_t)(len - 1) * c / chunks); };
+    for (uint32_t c = 0; c < chunks; ++c) {
+        EM_TRY(launch64(dev, static_cast<const double*>(b.charges), (uint32_t)p, dx, dy, dz,
+                        (uint32_t)n, bound(c + 1), n, sp, dsteps, b.stream, bound(c)));
+        EM_HIP(hipEventRecord(b.chunk_ev[c], b.stream));
+    }
+    EM_HIP(hipEventRecord(b.ev[2], b.stream));
+    double d2h = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const size_t r0 = bound(c), bytes = (size_t)(bound(c + 1) - r0) * row;
+        const size_t o = r0 * n;
+        const CopySeg segs[4] = {{x + o, dx + o, bytes}, {y + o, dy + o, bytes},
+                                 {z + o, dz + o, bytes}, {steps, dsteps, n * sizeof(uint32_t)}};
+        prefault_pages(segs, 3);  // while the GPU traces
+        EM_HIP(hipEventSynchronize(b.chunk_ev[c]));  // d2h_s: transfer time alone
+        EM_HIP(hipStreamWaitEvent(b.copy, b.chunk_ev[c], 0));
+        const double td = now64();
+        EM_TRY(d2h_staged(dev.device, segs, (steps && c + 1 == chunks) ? 4 : 3, b.copy));
+        d2h += now64() - td;
+        set_progress(perf, (double)bound(c + 1) / (double)len);
+    }
+    EM_HIP(hipEventSynchronize(b.ev[2]));
 
     float ms[2] = {0, 0};
-    EM_HIP(hipEventElapsedTime(&ms[0], r.ev[0], r.ev[1]));
-    EM_HIP(hipEventElapsedTime(&ms[1], r.ev[1], r.ev[2]));
+    EM_HIP(hipEventElapsedTime(&ms[0], b.ev[0], b.ev[1]));
+    EM_HIP(hipEventElapsedTime(&ms[1], b.ev[1], b.ev[2]));  // the kernels (chunks overlap copies)
     if (perf) {
         const double ks = ms[1] * 1e-3, dn = (double)n, dl = (double)len, dp = (double)p;
         perf->time_s = now64() - t0;
@@ -340,7 +460,7 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
         perf->line_steps_per_s = ks > 0 ? dn * (dl - 1.0) / ks : 0.0;
         // CPU_Implement.cpp:347-349
         perf->gflops = (dn * (dl - 1.0) * (dp * 18.0 + 13.0)) / perf->time_s / 1e9;
-        perf->lanes_per_line = 1;
+        perf->lanes_per_line = (int32_t)lanes64(dev, (uint32_t)n);
         perf->tile = (int32_t)(p <= kTile64 ? std::max<size_t>(p, 1) : kTile64 / 2);
         perf->block = kBlock64;
         perf->unroll = 0;  // the FP64 kernel's loop has no pipelined groups
@@ -386,7 +506,7 @@ int em_trace_dev_f64(const double* d_charges, size_t p, double* d_x, double* d_y
     if (pitch < n) EM_FAIL(EM_ERR_INVALID, "pitch %zu < n %zu", pitch, n);
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
-    return launch64(d_charges, (uint32_t)p, d_x, d_y, d_z, (uint32_t)n, (uint32_t)len, pitch, sp,
+    return launch64(dev, d_charges, (uint32_t)p, d_x, d_y, d_z, (uint32_t)n, (uint32_t)len, pitch, sp,
                     d_steps, static_cast<hipStream_t>(stream));
 }
 

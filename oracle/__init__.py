@@ -32,7 +32,8 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# EMTRACE_ORACLE_LIB: another build of the same sources (the sanitizer build)
+_LIB_PATH = os.environ.get("EMTRACE_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 _REF_PATH = os.path.join(HERE, "_ref", "libemref.so")
 REFERENCE_DIR = "/root/reference"
 

@@ -2,7 +2,10 @@
 
 * every row equals the oracle's FP64 restatement bit for bit -- plain, ref,
   angle with and without line ends, single- and multi-tile charge staging,
-  partial workgroups, pitch > n on the device form;
+  partial workgroups, one, two and four lanes per line, pitch > n on the
+  device form;
+* the host-array form traces the plain and ref rules in row chunks, bit-equal
+  to one device launch;
 * against the reference's own double rows (golden vectors) it is as close as
   the restatement is (-ffast-math reassociation in the reference build);
 * the drop-in calc_lines64 keeps the reference's status codes;
@@ -42,6 +45,10 @@ CASES = [
     ("ragged_block", 7, 130, 3.0, 50, rules.angle()),
     ("multi_tile_plain", 5000, 128, 20.0, 30, rules.plain(10.0)),
     ("multi_tile_angle", 4100, 192, 20.0, 40, rules.angle()),
+    # 32,768 < n < 65,536 lines: two lanes per line (the cases above take four,
+    # the 65,536-line tests one); 67 charges leave a last pass of one term
+    ("two_lanes_plain", 67, 40000, 10.0, 30, rules.plain(10.0)),
+    ("two_lanes_angle", 67, 40000, 5.0, 40, rules.angle()),
 ]
 
 
@@ -106,6 +113,29 @@ def test_fp64_vs_reference_double_golden(gpu, orc, path):
     rc, want = orc.calc_lines(g["charges"], g["seeds"], length, rule, dtype=np.float64)
     assert np.array_equal(got, want)
     assert np.abs(got - g["rows"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("rule", [rules.ref(1.0), rules.plain(1.0)], ids=["ref", "plain"])
+def test_fp64_host_path_row_chunks_match_device_path(gpu, rule):
+    """The reference's default problem (1,024 charges, 16,384 lines, 2,500
+    rows) in FP64 through the host-array entry point: 983 MB of output rows,
+    traced in eight row chunks whose copies back overlap the next chunk's
+    kernel.  Each chunk continues from the previous chunk's last row and the
+    seed stays row 0, so the rows and step counts are bit-identical to one
+    device-side launch; a second call reuses the cached device buffers."""
+    w = legacy.preset("default")
+    ch, sd = w.charges.astype(np.float64), w.seeds.astype(np.float64)
+    rows_d, steps_d = gpu.trace64(torch.from_numpy(ch).cuda(), torch.from_numpy(sd).cuda(), w.len,
+                                  rule, with_steps=True)
+    rows_d, steps_d = rows_d.cpu().numpy(), steps_d.cpu().numpy()
+    for call_no in range(2):
+        rows_h, perf, steps_h = gpu.trace_host64(ch, sd, w.len, rule)
+        print(f"[fp64 chunks {rule.rule}] call {call_no}: {perf['time_s'] * 1e3:.1f} ms, kernel "
+              f"{perf['kernel_s'] * 1e3:.1f} ms, d2h {perf['d2h_s'] * 1e3:.1f} ms", flush=True)
+        assert perf["progress"] == 1.0 and perf["kernel_s"] > 0
+        assert np.array_equal(steps_h.astype(np.int64), steps_d.astype(np.int64))
+        assert np.array_equal(rows_h, rows_d)
+        del rows_h
 
 
 def test_fp64_dropin_status_codes(gpu):

@@ -29,7 +29,9 @@ import pytest
 from emtrace import legacy, rules
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CLI = os.path.join(ROOT, "electromag-with-cuda_amd", "bin", "electromag")
+# EMTRACE_CLI: another build of the same command line (the sanitizer build)
+CLI = os.environ.get("EMTRACE_CLI") or os.path.join(ROOT, "electromag-with-cuda_amd", "bin",
+                                                    "electromag")
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 
