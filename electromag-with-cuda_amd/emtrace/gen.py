@@ -16,7 +16,8 @@ Workloads (sizes are those of the benchmark configurations):
   U[0.5, 1.0] with a random sign; every line seeded 0.1 from a uniformly chosen
   positive charge, direction on a Fibonacci sphere over the line index.
 * ``batched_configs(...)``          -- many independent configurations with
-  log-uniform charge and line counts, cube side 20*(p/1000)^(1/3).
+  log-uniform charge and line counts, cube side 20*(p/1000)^(1/3) for p
+  charges (the charge density of the 1,000-charge plain workload).
 * ``perturbed(charges, sigma, scale_lo, scale_hi)`` -- target charges for the
   inverse problem: positions + N(0, sigma^2), magnitudes * U[lo, hi].
 """
@@ -142,7 +143,13 @@ def log_uniform_int(seed: int, stream: int, count: int, lo: int, hi: int) -> np.
 
 
 def batched_configs(count: int, p_range, n_range, length: int, seed: int = 0):
-    """``count`` independent workloads; config i uses seed ``seed*1_000_003 + i + 1``."""
+    """``count`` independent workloads; config i uses seed ``seed*1_000_003 + i + 1``.
+
+    Configuration i's charges lie in a cube of side 20*(p_i/1000)^(1/3), p_i
+    being its charge count: the batched workload's description gives the side
+    as 20*(n/1000)^(1/3), and n is read as the charge count, so that every
+    configuration keeps the charge density of the 1,000-charge, side-20 plain
+    workload (1,000 charges in [-10, 10]^3) whatever its line count."""
     ps = log_uniform_int(seed, S_CFG_P, count, *p_range)
     ns = log_uniform_int(seed, S_CFG_N, count, *n_range)
     out = []
