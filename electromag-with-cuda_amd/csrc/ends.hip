@@ -44,8 +44,14 @@ constexpr int kSurvRec = 3;            // float4 records per set-aside line
 constexpr int kTailLanes = 256;        // lanes of a second-phase workgroup
 constexpr int kTailWaves = kTailLanes / 64;
 constexpr int kSlots = 8;              // lines a second-phase workgroup traces at once
-constexpr uint32_t kTailMinCharges = kEndsMinCharges;  // 4,096 = kTailLanes * kGroup
-static_assert(kTailMinCharges == kTailLanes * kGroup, "the second phase's bulk needs 2 groups");
+// Records per lane in each of the second phase's two pipelined register sets.
+// With 2 or 3 lines per workgroup each record feeds few interactions, and
+// eight sets of the unrolled sum (one per active-line count) share the
+// instruction cache: on the 16,384-charge, 65,536-line problem 4 records take
+// 20.6 ms, against 21.9 ms with 8, 23.6 ms with 16 and 21.1 ms with 2.
+constexpr int kTailU = 4;
+constexpr uint32_t kTailMinCharges = kEndsMinCharges;  // 4,096
+static_assert(kTailMinCharges >= 2u * kTailLanes * kTailU, "the second phase's bulk needs 2 groups");
 constexpr size_t kTailReserve = 64 * 1024;  // LDS that keeps two second-phase workgroups per CU
 constexpr uint32_t kFillRows = 64;     // rows per fill thread
 
@@ -202,8 +208,8 @@ __device__ __forceinline__ float4 buffer_record(__amdgpu_buffer_rsrc_t rsrc, uin
 // records (a multiple of kTailLanes, at least kTailMinCharges) split over the
 // kTailLanes lanes: every record a lane loads serves all NACT points.  The
 // bulk runs the software pipeline of pipelined_tile (two register sets of
-// kUnroll records, one set's loads in flight while the other is consumed),
-// reading global memory; the last pp mod (2 kUnroll kTailLanes) records
+// kTailU records, one set's loads in flight while the other is consumed),
+// reading global memory; the last pp mod (2 kTailU kTailLanes) records
 // follow one per lane at a time.  Each lane
 // sums pp / kTailLanes records per point in plain FP32 (the blocked
 // compensation of the other kernels folds every 128 records, so it would
@@ -214,7 +220,7 @@ template <int NACT>
 __device__ __forceinline__ void tail_sums(__amdgpu_buffer_rsrc_t rsrc, uint32_t pp, int tid,
                                           const float4* __restrict__ pts, float4* __restrict__ part)
 {
-    constexpr uint32_t kStride = kTailLanes * kUnroll;  // records per group
+    constexpr uint32_t kStride = kTailLanes * kTailU;  // records per group
     const uint32_t ngroups = (pp / kStride) & ~1u;  // even, >= 2 since pp >= kTailMinCharges
     const uint32_t lane_off = (uint32_t)tid * 16u;
     float px[NACT], py[NACT], pz[NACT], ex[NACT], ey[NACT], ez[NACT];
@@ -224,25 +230,25 @@ __device__ __forceinline__ void tail_sums(__amdgpu_buffer_rsrc_t rsrc, uint32_t 
         px[l] = q.x; py[l] = q.y; pz[l] = q.z;
         ex[l] = ey[l] = ez[l] = 0.f;
     }
-    float4 a[kUnroll], b[kUnroll];
+    float4 a[kTailU], b[kTailU];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) a[u] = buffer_record(rsrc, lane_off, u * kTailLanes);
+    for (int u = 0; u < kTailU; ++u) a[u] = buffer_record(rsrc, lane_off, u * kTailLanes);
     for (uint32_t g = 0; g < ngroups; g += 2) {
         const uint32_t rb = (g + 1) * kStride;
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) b[u] = buffer_record(rsrc, lane_off, rb + u * kTailLanes);
+        for (int u = 0; u < kTailU; ++u) b[u] = buffer_record(rsrc, lane_off, rb + u * kTailLanes);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < kTailU; ++u)
 #pragma unroll
             for (int l = 0; l < NACT; ++l) coulomb(a[u], px[l], py[l], pz[l], ex[l], ey[l], ez[l]);
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t ra = min(g + 2, ngroups - 2) * kStride;  // the last pair re-reads itself
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) a[u] = buffer_record(rsrc, lane_off, ra + u * kTailLanes);
+        for (int u = 0; u < kTailU; ++u) a[u] = buffer_record(rsrc, lane_off, ra + u * kTailLanes);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < kTailU; ++u)
 #pragma unroll
             for (int l = 0; l < NACT; ++l) coulomb(b[u], px[l], py[l], pz[l], ex[l], ey[l], ez[l]);
         __builtin_amdgcn_sched_barrier(0);
@@ -385,6 +391,16 @@ __global__ void __launch_bounds__(kTailLanes) k_trace_tail(Cfg cfg, const float4
         __syncthreads();  // pts and s_nact published
         const uint32_t n_now = __builtin_amdgcn_readfirstlane(s_nact);
         if (n_now == 0u) break;
+        // A CU's two workgroups hold unequal line counts (3 and 2 where it has
+        // 5 lines): the waves of the one with more lines issue first, and the
+        // other fills their stalls, so both end closer together (19.7 ms
+        // against 20.6 ms at equal priority on the problem above).
+        if (n_now >= 3u)
+            __builtin_amdgcn_s_setprio(3);
+        else if (n_now == 2u)
+            __builtin_amdgcn_s_setprio(2);
+        else
+            __builtin_amdgcn_s_setprio(1);
         switch (n_now) {
             case 1: tail_sums<1>(rsrc, pp, tid, pts, part); break;
             case 2: tail_sums<2>(rsrc, pp, tid, pts, part); break;
