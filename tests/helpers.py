@@ -89,6 +89,11 @@ def rows64_to_soa(rows64: np.ndarray) -> np.ndarray:
 #       times the larger of the two (a few random draws sample a
 #       distribution whose tail is wider; systematic rounding bias grows
 #       faster than random signs).
+# For the plain and ref rules an eligible line (one that stays more than 0.05
+# from every charge) is exempted only by d_cpu, the literal rule's own reading;
+# for the angle rule sens may exempt it too (the adaptive step's length is a
+# rounding-sensitive function of the turn, so FP64 itself can move far more
+# than the CPU path happens to), and those lines are counted and printed.
 # Every exemption is printed with its numbers; a line meeting neither fails.
 # Over all lines the GPU's distance to FP64 is no worse than twice the CPU
 # path's (median and 90th percentile).
@@ -140,11 +145,14 @@ def fp64_sensitivity(charges, seeds, length, rule, first, lines, trials=4, rel=2
 
 
 def line_gates(tag, g_rows, c_rows, r64, charges, seeds, rule, tol=TOL, cond=COND,
-               min_conditioned=0.5, factor=SPREAD_FACTOR):
+               min_conditioned=0.5, factor=SPREAD_FACTOR, strict_eligible=None):
     """Apply the per-line rule above; print every exempted line and every
-    failing one; returns a summary dict."""
+    failing one; returns a summary dict.  strict_eligible (default: not the
+    angle rule): eligible lines are exempted only by the CPU path's departure."""
     from emtrace import rules as R
     use_path = rule.rule == R.ANGLE
+    if strict_eligible is None:
+        strict_eligible = rule.rule != R.ANGLE
     assert np.isfinite(g_rows).all(), f"{tag}: non-finite GPU points"
     first, eligible = prefix_lengths(c_rows, charges)
     e_gc = _per_line(g_rows, c_rows, first, use_path)
@@ -156,6 +164,8 @@ def line_gates(tag, g_rows, c_rows, r64, charges, seeds, rule, tol=TOL, cond=CON
     sens[need] = fp64_sensitivity(charges, seeds, c_rows.shape[1], rule, first, need)
     scale = np.maximum(e_c64, sens)
     exempt = ~ok & (scale > cond) & (e_gc <= factor * scale)
+    if strict_eligible:
+        exempt &= ~eligible | (e_c64 > cond)
     failing = np.nonzero(~ok & ~exempt)[0]
     conditioned = e_c64 <= cond
     n = len(ok)
@@ -169,6 +179,7 @@ def line_gates(tag, g_rows, c_rows, r64, charges, seeds, rule, tol=TOL, cond=CON
         "exempt_eligible": int((exempt & eligible).sum()),
         "exempt_by_cpu_departure": int((exempt & (e_c64 > cond)).sum()),
         "exempt_by_sensitivity_only": int((exempt & (e_c64 <= cond)).sum()),
+        "exempt_eligible_by_sensitivity_only": int((exempt & eligible & (e_c64 <= cond)).sum()),
         "exempt_min_scale": float(scale[exempt].min()) if exempt.any() else None,
         "exempt_max_gap_over_scale": float((e_gc[exempt] / scale[exempt]).max()) if exempt.any() else None,
         "failing": int(failing.size),
