@@ -120,7 +120,7 @@ int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch*
 {
     Shape s;
     if (refill) {
-        // first phase of the two-phase line-ends trace: a group takes a new
+        // first phase of the lane-group line-ends trace: a group takes a new
         // line as soon as its line ends, so blocks stay full and may be
         // large; wide groups keep the slowest line of the phase short
         int S = 1;
@@ -339,7 +339,7 @@ static int host_trace(const float* charges, size_t p, float* x, float* y, float*
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    // line ends (no checkpoints here) over many charges: the two-phase trace
+    // line ends (no checkpoints here) over many charges: launch_trace_ends
     const bool ends = st->stop != 0 && p >= kEndsMinCharges;
     EM_TRY(choose_shape(dev, p, n, launch, sh, st->stop != 0, ends));
 
@@ -505,7 +505,7 @@ static int trace_dev(const float* d_charges, size_t p, float* d_x, float* d_y, f
     DeviceInfo dev;
     EM_TRY(current_device_info(dev));
     Shape sh;
-    // line ends without checkpoints over many charges: the two-phase trace;
+    // line ends without checkpoints over many charges: launch_trace_ends;
     // with checkpoints the one-kernel path, whose steps the adjoint's
     // recompute repeats
     const bool ends = step->stop && !d_ckpt && p >= kEndsMinCharges;

@@ -74,7 +74,8 @@ struct Shape {
 // unsupported request.
 // `stop`: the angle rule with line ends (favours wide line groups and small
 // blocks); the forward and the adjoint of one problem must choose alike.
-// `refill`: the first phase of the two-phase line-ends trace (launch_trace_ends).
+// `refill`: the first phase of the lane-group line-ends trace (launch_trace_ends,
+// above 16,384 charges; the register-resident trace takes no shape).
 int choose_shape(const DeviceInfo& dev, uint64_t p, uint64_t n, const em_launch* req, Shape& out,
                  bool stop = false, bool refill = false);
 int validate_shape(const DeviceInfo& dev, const Shape& s);
@@ -120,10 +121,12 @@ int persistent_grid(const Shape& sh, const DeviceInfo& dev, int* grid);
 // steps are short sums, through the one-kernel path.
 constexpr uint32_t kEndsMinCharges = 4096;
 
-// The angle rule with line ends and no checkpoints, in two phases (ends.hip):
-// lane groups that take a new line whenever theirs ends, up to a step cap;
-// then one 256-lane workgroup per line still moving; then a coalesced fill of
-// the rows after each line's end.  `steps` may be NULL.
+// The angle rule with line ends and no checkpoints (ends.hip): up to 16,384
+// charges the register-resident trace of wide.hip (1,024-lane workgroups
+// holding every charge record, three passes); above, lane groups that take a
+// new line whenever theirs ends, up to a step cap, then 256-lane workgroups
+// for the lines still moving.  Both end with a coalesced fill of the rows
+// after each line's end.  `steps` may be NULL.
 int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
                       float* x, float* y, float* z, const StepP& sp, uint32_t* steps,
                       hipStream_t stream);

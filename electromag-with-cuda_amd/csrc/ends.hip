@@ -1,4 +1,7 @@
-// ends.hip -- the angle rule with line ends, traced in two phases.
+// ends.hip -- the angle rule with line ends: the entry points, their work
+// memory, and the lane-group trace for problems above 16,384 charges (4,096
+// to 16,384 charges take the register-resident trace of wide.hip; fewer, the
+// one-kernel path of trace.hip).
 //
 // With line ends most lines of a problem are short (a line that runs into a
 // sink ends there) and a few never end (they escape and take every row).  In
@@ -6,7 +9,7 @@
 // while 1.8 % of the lines run all 2,499 and hold 46 % of the steps.  One
 // launch shape cannot serve both: groups of S lanes per line keep a wave busy
 // until its slowest line ends, and a line that never ends is then traced at
-// the rate of S lanes.  Here:
+// the rate of S lanes.  The lane-group trace, in two phases:
 //   1. k_trace_ends<S, U>: a persistent grid of S-lane groups.  A group whose
 //      line ends takes the next line from a counter; a line still moving
 //      after kEndsCap steps is set aside (its state goes to a list) and the
@@ -33,6 +36,7 @@
 #include <vector>
 
 #include "em_internal.hpp"
+#include "wide.hpp"
 
 namespace em {
 
@@ -524,8 +528,10 @@ int dispatch_phase1(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, cons
 #undef EM_PHASE1_U
 }
 
-// Per-device work memory of the two-phase trace: counters, step counts (when
-// the caller keeps none), the set-aside list and the padded charge copy.
+// Per-device work memory of the line-ends trace: counters, step counts (when
+// the caller keeps none), and the register-resident trace's two lists and
+// their orders, or the lane-group trace's set-aside list and padded charge
+// copy.
 // Eager calls on one device use it in turn: each call's work waits for the
 // previous eager call's (an event).  A captured graph uses the buffer current
 // at capture; its replays are ordered only by the caller's streams, so a
@@ -546,16 +552,31 @@ EndsScratch g_ends[64];
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 // Where the pieces of the work memory lie for one problem, and how much it
-// needs.
+// needs.  Problems of 4,096..16,384 charges take the register-resident trace
+// of wide.hip (`wide`); larger ones the two phases here.
 struct EndsLayout {
+    bool wide;        // the wide trace (wide.hip)
+    WideLayout w;
     bool tail;        // the second phase runs
     uint32_t cap, drain, pp;
     size_t off_steps, off_surv, off_pad, need;
 };
 
-EndsLayout ends_layout(const Cfg& cfg, bool own_steps)
+EndsLayout ends_layout(const DeviceInfo& dev, const Cfg& cfg, bool own_steps)
 {
     EndsLayout L;
+    L.wide = wide_applies(cfg);
+    if (L.wide) {
+        L.tail = false;
+        L.cap = L.drain = cfg.len;
+        L.pp = 0;
+        L.off_steps = align256(sizeof(EndsCtrl) > sizeof(WideCtrl) ? sizeof(EndsCtrl) : sizeof(WideCtrl));
+        const size_t steps_bytes = own_steps ? align256((size_t)cfg.n * sizeof(uint32_t)) : 0;
+        L.w = wide_layout(dev, L.off_steps + steps_bytes);
+        L.off_surv = L.off_pad = L.w.end;
+        L.need = L.w.end;
+        return L;
+    }
     // The second phase pays off when a line's step is a long sum.  Below
     // kTailMinCharges charges the first phase's wide groups take every line
     // to its end (a step is then at most 128 records per lane), and no
@@ -586,7 +607,7 @@ int ends_check(const DeviceInfo& dev, const Cfg& cfg)
     return EM_OK;
 }
 
-// The launches of one two-phase trace over work memory `buf` (laid out by
+// The launches of one line-ends trace over work memory `buf` (laid out by
 // `L`).  Allocates nothing; the counters are zeroed by a kernel, so the
 // sequence may be captured into a graph.
 int ends_launch(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const float4* charges,
@@ -597,6 +618,13 @@ int ends_launch(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const fl
     uint32_t* st = steps ? steps : reinterpret_cast<uint32_t*>(buf + L.off_steps);
     float4* surv = reinterpret_cast<float4*>(buf + L.off_surv);
     float4* padded = reinterpret_cast<float4*>(buf + L.off_pad);
+    const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
+    if (L.wide) {
+        EM_TRY(wide_launch(dev, cfg, charges, x, y, z, sp, st, stream, buf, L.w));
+        hipLaunchKernelGGL(k_ends_fill, fill_grid, dim3(256), 0, stream, cfg, x, y, z, st);
+        EM_HIP(hipGetLastError());
+        return EM_OK;
+    }
 
     hipLaunchKernelGGL(k_ends_prepare, dim3(std::max<uint32_t>(1u, (L.pp + 255) / 256)), dim3(256), 0,
                        stream, ctrl, charges, cfg.p, padded, L.pp);
@@ -620,7 +648,6 @@ int ends_launch(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const fl
         EM_TRY(dispatch_phase1<true>(sh, dev, cfg, charges, x, y, z, sp, cfg.len, cfg.len,
                                      tail_max, st, ctrl, surv, stream));
     }
-    const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
     hipLaunchKernelGGL(k_ends_fill, fill_grid, dim3(256), 0, stream, cfg, x, y, z, st);
     EM_HIP(hipGetLastError());
     return EM_OK;
@@ -661,7 +688,7 @@ int launch_trace_ends_noalloc(const Shape& sh, const DeviceInfo& dev, const Cfg&
                               uint32_t* steps, hipStream_t stream)
 {
     EM_TRY(ends_check(dev, cfg));
-    const EndsLayout L = ends_layout(cfg, steps == nullptr);
+    const EndsLayout L = ends_layout(dev, cfg, steps == nullptr);
     if (L.need > kNoAllocScratchBytes)
         EM_FAIL(EM_ERR_INVALID,
                 "line ends without allocation: %u lines and %u charges need %zu bytes of work "
@@ -676,7 +703,7 @@ int launch_trace_ends(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, co
                       hipStream_t stream)
 {
     EM_TRY(ends_check(dev, cfg));
-    const EndsLayout L = ends_layout(cfg, steps == nullptr);
+    const EndsLayout L = ends_layout(dev, cfg, steps == nullptr);
     const size_t need = L.need;
 
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;

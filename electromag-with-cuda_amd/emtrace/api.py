@@ -33,6 +33,20 @@ def make_launch(lanes_per_line: int = 0, tile: int = 0, block: int = 0,
     return em_launch(int(lanes_per_line), int(tile), int(block), int(unroll))
 
 
+# Line ends (angle rule with stop) without checkpoints over ENDS_MIN_CHARGES to
+# WIDE_MAX_CHARGES charges take the register-resident trace, which has one
+# fixed launch: a launch shape given for it is not used.
+ENDS_MIN_CHARGES = 4096
+WIDE_MAX_CHARGES = 16384
+
+
+def takes_launch_shape(p: int, rule, ckpt: bool = False) -> bool:
+    """Whether a single-configuration trace of p charges with ``rule`` uses a
+    launch shape (all but the register-resident line-ends trace)."""
+    return not (rule.stop and rule.rule == 2 and not ckpt
+                and ENDS_MIN_CHARGES <= p <= WIDE_MAX_CHARGES)
+
+
 def _launch_arg(launch):
     if launch is None:
         return None

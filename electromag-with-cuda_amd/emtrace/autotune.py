@@ -75,13 +75,11 @@ def _device_key() -> str:
 # local cache has no entry: the fastest shape of each sweep, ties (within
 # 1%) going to fewer lanes per line.  [S, block, tile, U, ms at 48 rows]
 SHIPPED = {
-    # 16,384 charges, 65,536 lines, angle rule: all steps (2,500 rows: 831 ms;
-    # unroll 4: 847 ms, 8: 917 ms; 256-thread blocks: 984 ms at best); with
-    # line ends the first phase of the two-phase trace (49-51 ms in all, any
-    # unroll within 2 %; with 16 lanes per line 53-57 ms, with 8 lanes 76 ms,
-    # with one-wave blocks over 512-charge tiles 70 ms)
+    # 16,384 charges, 65,536 lines, angle rule, all steps (2,500 rows: 831 ms;
+    # unroll 4: 847 ms, 8: 917 ms; 256-thread blocks: 984 ms at best).  With
+    # line ends this problem takes the register-resident trace, which uses no
+    # launch shape (tune_plain does not time it).
     "plain:gfx950:256:16384:65536:2:0": [4, 512, 4096, 2, None],
-    "plain:gfx950:256:16384:65536:2:1": [32, 512, 4096, 8, None],
     # 512 configurations, up to 8,192 charges, 713,262 lines, angle rule
     # (1,000 rows: 432 ms with 2,048-charge tiles, four workgroups per CU; 8
     # lanes per line within 1 %; 4,096-charge tiles, two workgroups per CU:
@@ -154,10 +152,10 @@ def candidates(p: int, n: int, cus: int, batch: bool = False,
     Single configuration: S grows until the grid has 8 waves per SIMD.  Batch:
     S in {1, 2, 4, 8} always -- a smaller line block per work item evens out
     the persistent grid's finishing times even when waves are plentiful.
-    With line ends (stop) the shape is that of the first phase of the
-    two-phase trace, where a lane group takes a new line as soon as its line
-    ends: S from 4 up to 32 (wide groups shorten the phase's last lines),
-    blocks from one wave up, and smaller tiles as well."""
+    With line ends (stop) and more than 16,384 charges the shape is that of
+    the first phase of the lane-group trace, where a lane group takes a new
+    line as soon as its line ends: S from 4 up to 32 (wide groups shorten the
+    phase's last lines), blocks from one wave up, and smaller tiles as well."""
     out = []
     for S in (1, 2, 4, 8, 16, 32):
         if batch:
@@ -215,6 +213,8 @@ def tune_plain(charges, seeds, rule: StepRule, rows: Optional[int] = None, use_c
     lists (S, block, tile, U, ms); U = 0 is the library's default unroll."""
     import torch
     p, n = int(charges.shape[0]), int(seeds.shape[0])
+    if not api.takes_launch_shape(p, rule):
+        return api.make_launch(), [("register-resident line-ends trace: no launch shape",)]
     if rows is None:
         rows = 400 if rule.stop else 48
     key = f"plain:{_device_key()}:{_bucket(p)}:{_bucket(n)}:{rule.rule}:{int(rule.stop)}"

@@ -166,16 +166,20 @@ size_t em_ckpt_floats(size_t n, size_t len, uint32_t ckpt_every);
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory: code under stream/graph capture, tight serving loops, harnesses that
  * forbid hipMalloc.  Same arguments, same rows and step counts as
- * em_trace_dev_f32 (the same kernels), but the work memory of the two-phase
- * line-ends trace is a fixed array in the library's device code,
+ * em_trace_dev_f32 (the same kernels), but the work memory of the line-ends
+ * trace is a fixed array in the library's device code,
  * EM_NOALLOC_SCRATCH_BYTES per device, held from the first call on a device
  * (or em_init) until the process ends.  The call allocates nothing, creates
  * no stream or event, and never waits on the host.
  * Work memory needed by that path (angle rule with stop, no checkpoints,
- * p >= 4096): 256 + a(48 n) + 16 round_up(p, 256) bytes, plus a(4 n) when
- * d_steps is NULL, where a() rounds up to 256; more than
- * EM_NOALLOC_SCRATCH_BYTES -> EM_ERR_INVALID before any launch (with steps
- * given: about 170,000 lines at 16,384 charges).  The other paths use no work
+ * p >= 4096), where a() rounds up to 256 and C is the device's CU count:
+ *   4096 <= p <= 16384 (register-resident trace): 256 + 104 * 16 C bytes
+ *     (425,984 + 256 on 256 CUs, whatever n), plus a(4 n) when d_steps is NULL;
+ *   p > 16384 (lane-group trace): 256 + a(48 n) + 16 round_up(p, 256) bytes,
+ *     plus a(4 n) when d_steps is NULL.
+ * More than EM_NOALLOC_SCRATCH_BYTES -> EM_ERR_INVALID before any launch
+ * (on 256 CUs: about 1,990,000 lines without a steps buffer up to 16,384
+ * charges; about 170,000 lines with one above).  The other paths use no work
  * memory.
  * Every call on one device uses that device's one array: calls that overlap
  * on a device (on different streams, or a graph replay beside another call)

@@ -2,6 +2,8 @@
 tie-break, and the shipped MI355X table."""
 import math
 
+import numpy as np
+
 from emtrace import autotune
 
 
@@ -74,3 +76,22 @@ def test_stop_candidates_include_wide_groups_and_small_blocks():
     stop = autotune.candidates(16384, 65536, 256, stop=True)
     assert max(c[0] for c in full) <= 8  # enough waves already at S = 4
     assert (32, 64, 512) in stop and min(c[1] for c in stop) == 64
+
+
+def test_register_resident_trace_takes_no_launch_shape():
+    """Line ends without checkpoints over 4,096..16,384 charges take the
+    register-resident trace, whose launch is fixed: the tuner does not time
+    shapes for it (the library ignores a given one)."""
+    from emtrace import api, rules
+    assert not api.takes_launch_shape(4096, rules.angle())
+    assert not api.takes_launch_shape(16384, rules.angle())
+    assert api.takes_launch_shape(4095, rules.angle())
+    assert api.takes_launch_shape(16385, rules.angle())
+    assert api.takes_launch_shape(16384, rules.angle(stop=False))
+    assert api.takes_launch_shape(16384, rules.angle(), ckpt=True)
+    assert api.takes_launch_shape(16384, rules.plain())
+    charges = np.zeros((8192, 4), np.float32)
+    seeds = np.zeros((64, 3), np.float32)
+    launch, report = autotune.tune_plain(charges, seeds, rules.angle())
+    assert (launch.lanes_per_line, launch.tile, launch.block, launch.unroll) == (0, 0, 0, 0)
+    assert "no launch shape" in report[0][0]

@@ -42,8 +42,8 @@ def forward_and_adjoint(em, charges, seeds, length, rule, G, every):
 
 def forward_steps(em, charges, seeds, length, rule, every=50):
     """Step counts of the checkpointed forward (the one-kernel path that the
-    adjoint's recompute repeats; without checkpoints, line ends are traced in
-    two phases whose charge sums are split differently past step 192)."""
+    adjoint's recompute repeats; without checkpoints, line ends of many charges
+    are traced by kernels whose charge sums are split differently)."""
     n = seeds.shape[0]
     rows = em.seed_rows(dev(seeds), length)
     ck = torch.empty(max(1, em.ckpt_size(n, length, every)), dtype=torch.float32, device="cuda")

@@ -43,7 +43,7 @@ def test_drop_in_calls_from_threads(gpu):
 
 
 def test_device_calls_on_streams_from_threads(gpu):
-    """Line ends on the two-phase path (4,500 charges: shared work memory) and
+    """Line ends on the register-resident trace (4,500 charges: shared work memory) and
     all-steps traces, each thread on its own stream."""
     ws = [gen.plain_config(4500, 600, 12.0, 260, seed=50 + i) for i in range(3)]
     ws += [gen.plain_config(800, 900, 10.0, 150, seed=60 + i) for i in range(3)]

@@ -50,9 +50,10 @@ def test_all_steps_identical_for_every_unroll(gpu, S, block):
 
 @pytest.mark.parametrize("p", [900, 5000])
 def test_line_ends_identical_for_every_unroll(gpu, p):
-    """Below 4,096 charges the one-kernel path, above it the two-phase path
-    (first-phase shape from the launch); hmax 0.05 so that lines end and some
-    cross the first phase's 192-step cap within 400 rows."""
+    """Below 4,096 charges the one-kernel path, above it the register-resident
+    trace (which takes no launch shape: its rows are the same for every
+    unroll by construction); hmax 0.05 so that lines end and some outlast its
+    settling pass within 400 rows."""
     w = gen.plain_config(p, 1200, 10.0, 400, seed=22)
     rule = rules.angle(hmax=0.05)
     base = trace_all(gpu, w.charges, w.seeds, w.len, rule, gpu.make_launch(32, 0, 512, 8))
