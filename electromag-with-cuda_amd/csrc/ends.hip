@@ -619,12 +619,7 @@ int ends_launch(const Shape& sh, const DeviceInfo& dev, const Cfg& cfg, const fl
     float4* surv = reinterpret_cast<float4*>(buf + L.off_surv);
     float4* padded = reinterpret_cast<float4*>(buf + L.off_pad);
     const dim3 fill_grid((cfg.n + 255) / 256, (cfg.len + kFillRows - 1) / kFillRows);
-    if (L.wide) {
-        EM_TRY(wide_launch(dev, cfg, charges, x, y, z, sp, st, stream, buf, L.w));
-        hipLaunchKernelGGL(k_ends_fill, fill_grid, dim3(256), 0, stream, cfg, x, y, z, st);
-        EM_HIP(hipGetLastError());
-        return EM_OK;
-    }
+    if (L.wide) return wide_launch(dev, cfg, charges, x, y, z, sp, st, stream, buf, L.w);
 
     hipLaunchKernelGGL(k_ends_prepare, dim3(std::max<uint32_t>(1u, (L.pp + 255) / 256)), dim3(256), 0,
                        stream, ctrl, charges, cfg.p, padded, L.pp);

@@ -59,7 +59,49 @@ struct WideShared {
     float4 slot[2 * kGroupMax][3];             // wave 0's line states (see slot_load)
     uint32_t cnt[2];                           // lines of each group
     uint32_t exit[2];                          // written in half-step h at [h & 1]
+    uint32_t fill_k;                           // FINISH: this workgroup's next fill tile
 };
+
+// ---- rows after a line's end (FINISH fills them for the lines that ended before it) ----
+// A tile is 64 consecutive lines x kWideFillRows rows; workgroup b of G takes
+// tiles b, b + G, ...  Before FINISH the lines of list 2 (still moving) get
+// kUnended in steps[], and the tiles skip them.  A list-2 line that ends
+// during FINISH has its step written then, and a tile read after that may
+// fill its rows from an end point not yet visible across the chip's L2s;
+// k_wide_fill_list rewrites every list-2 line's rows after FINISH, so the
+// rows end the same either way.
+constexpr uint32_t kWideFillRows = 16;
+constexpr uint32_t kUnended = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint64_t fill_tiles(const Cfg& cfg)
+{
+    return (uint64_t)((cfg.n + 63u) / 64u) * ((cfg.len + kWideFillRows - 1u) / kWideFillRows);
+}
+
+// Rows of tile t after each (ended) line's end get the line's end point.
+__device__ __forceinline__ void fill_tile(const Cfg& cfg, float* __restrict__ X, float* __restrict__ Y,
+                                          float* __restrict__ Z, const uint32_t* __restrict__ steps,
+                                          uint64_t t, int lane)
+{
+    const uint32_t nb = (cfg.n + 63u) / 64u;
+    const uint32_t line = (uint32_t)(t % nb) * 64u + (uint32_t)lane;
+    const uint32_t rc = (uint32_t)(t / nb);
+    if (line >= cfg.n) return;
+    const uint32_t e = steps[cfg.line_index + line];
+    if (e == kUnended) return;
+    const uint32_t r0 = max(e + 1u, rc * kWideFillRows);
+    const uint32_t r1 = min(cfg.len, (rc + 1u) * kWideFillRows);
+    if (r0 >= r1) return;
+    const uint64_t base = cfg.line_off + line;
+    const uint64_t src = base + (uint64_t)e * cfg.pitch;
+    const float x = X[src], y = Y[src], z = Z[src];
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint64_t o = base + (uint64_t)r * cfg.pitch;
+        X[o] = x;
+        Y[o] = y;
+        Z[o] = z;
+    }
+}
 
 // One line's state as wave 0 keeps it in LDS: {x, y, z, h}, {ux, uy, uz, line},
 // {s (next step), steps taken, steps taken in this pass, 0}.
@@ -252,7 +294,7 @@ __global__ void __launch_bounds__(kWideLanes) k_wide(Cfg cfg, const float4* __re
                                                      uint32_t in_cap, float4* __restrict__ list_out,
                                                      uint32_t* __restrict__ count_out,
                                                      uint32_t out_cap, uint32_t cap,
-                                                     uint64_t iter_max)
+                                                     uint64_t iter_max, uint32_t fill)
 {
     __shared__ WideShared sh;
 
@@ -364,6 +406,7 @@ __global__ void __launch_bounds__(kWideLanes) k_wide(Cfg cfg, const float4* __re
         if (lane == 0) {
             sh.exit[0] = 0u;
             sh.exit[1] = got == 0u ? 1u : 0u;  // read at half-step 0
+            sh.fill_k = 0u;
         }
         if (MODE == kFresh && lane == 0)
             flag_v = __hip_atomic_load(&ctrl->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -479,8 +522,51 @@ __global__ void __launch_bounds__(kWideLanes) k_wide(Cfg cfg, const float4* __re
         // ---- every wave: the sums of group grp ----
         wide_sums_all<R>(rec, sh.pts[grp], sh.part[grp], n, lane, wave);
         if (grp) { pd1 = n > 0u; pd0 = false; } else { pd0 = n > 0u; pd1 = false; }
+        // FINISH: wave 1 (the first of its SIMD to end its sums) fills one tile
+        if (MODE == kFinish && fill && __builtin_amdgcn_readfirstlane(wave) == 1) {
+            const uint32_t k = __builtin_amdgcn_readfirstlane(sh.fill_k);
+            const uint64_t t = b + (uint64_t)k * G;
+            if (t < fill_tiles(cfg)) {
+                fill_tile(cfg, X, Y, Z, steps, t, lane);
+                if (lane == 0) sh.fill_k = k + 1u;
+            }
+        }
         __syncthreads();
         grp = og;
+    }
+    // FINISH: every wave fills this workgroup's tiles left
+    if (MODE == kFinish && fill) {
+        __syncthreads();
+        const uint64_t k0 = __builtin_amdgcn_readfirstlane(sh.fill_k);
+        const uint64_t nt = fill_tiles(cfg);
+        for (uint64_t t = b + (k0 + (uint32_t)wave) * G; t < nt; t += (uint64_t)kWideWaves * G)
+            fill_tile(cfg, X, Y, Z, steps, t, lane);
+    }
+}
+
+// Rows after the end of each line of `list` (the lines FINISH traced): lanes
+// are (entry, row chunk) pairs.
+__global__ void k_wide_fill_list(Cfg cfg, float* __restrict__ X, float* __restrict__ Y,
+                                 float* __restrict__ Z, const uint32_t* __restrict__ steps,
+                                 const float4* __restrict__ list, const uint32_t* __restrict__ count,
+                                 uint32_t cap)
+{
+    const uint32_t m = min(__builtin_amdgcn_readfirstlane(*count), cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t line = __float_as_uint(list[3 * (uint64_t)i + 1].w);
+    const uint32_t e = steps[cfg.line_index + line];
+    const uint32_t r0 = max(e + 1u, blockIdx.y * 64u);
+    const uint32_t r1 = min(cfg.len, (blockIdx.y + 1u) * 64u);
+    if (e == kUnended || r0 >= r1) return;
+    const uint64_t base = cfg.line_off + line;
+    const uint64_t src = base + (uint64_t)e * cfg.pitch;
+    const float x = X[src], y = Y[src], z = Z[src];
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint64_t o = base + (uint64_t)r * cfg.pitch;
+        X[o] = x;
+        Y[o] = y;
+        Z[o] = z;
     }
 }
 
@@ -490,11 +576,14 @@ __global__ void __launch_bounds__(kWideLanes) k_wide(Cfg cfg, const float4* __re
 __global__ void __launch_bounds__(1024) k_wide_order(const float4* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
                                                      uint32_t* __restrict__ order, uint32_t cap,
-                                                     int desc)
+                                                     int desc, uint32_t* __restrict__ mark)
 {
     __shared__ uint64_t key[kWideSortMax];
     const uint32_t m = min(__builtin_amdgcn_readfirstlane(*count), cap);
     const uint32_t tid = threadIdx.x;
+    // `mark` (steps + line_index) non-null: the list's lines have not ended
+    if (mark)
+        for (uint32_t i = tid; i < m; i += blockDim.x) mark[__float_as_uint(list[3 * (uint64_t)i + 1].w)] = kUnended;
     if (m > kWideSortMax) {
         for (uint32_t i = tid; i < m; i += blockDim.x) order[i] = i;
         return;
@@ -560,21 +649,26 @@ int wide_launch_r(const DeviceInfo& dev, const Cfg& cfg, const float4* charges, 
     EM_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_wide<R, kFresh>), dim3(G), dim3(kWideLanes), 0, stream, cfg, charges, x, y,
                        z, sp, steps, ctrl, gs, nullptr, nullptr, nullptr, 0u, l1, &ctrl->count1,
-                       L.list_cap, 0u, iter_max);
+                       L.list_cap, 0u, iter_max, 0u);
     EM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(1024), 0, stream, l1, &ctrl->count1, o1,
-                       L.list_cap, 1);
+                       L.list_cap, 1, (uint32_t*)nullptr);
     EM_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_wide<R, kSettle>), dim3(G), dim3(kWideLanes), 0, stream, cfg, charges, x,
                        y, z, sp, steps, ctrl, gs, l1, o1, &ctrl->count1, L.list_cap, l2,
-                       &ctrl->count2, L.list_cap, kWideSettle, iter_max);
+                       &ctrl->count2, L.list_cap, kWideSettle, iter_max, 0u);
     EM_HIP(hipGetLastError());
+    // list 2's lines are still moving: FINISH's fill leaves them to k_wide_fill_list
     hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(1024), 0, stream, l2, &ctrl->count2, o2,
-                       L.list_cap, 0);
+                       L.list_cap, 0, steps + cfg.line_index);
     EM_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_wide<R, kFinish>), dim3(G), dim3(kWideLanes), 0, stream, cfg, charges, x,
                        y, z, sp, steps, ctrl, gs, l2, o2, &ctrl->count2, L.list_cap, nullptr,
-                       nullptr, 0u, 0u, iter_max);
+                       nullptr, 0u, 0u, iter_max, 1u);
+    EM_HIP(hipGetLastError());
+    const dim3 fill_grid((L.list_cap + 255) / 256, (cfg.len + 63) / 64);
+    hipLaunchKernelGGL(k_wide_fill_list, fill_grid, dim3(256), 0, stream, cfg, x, y, z, steps, l2,
+                       &ctrl->count2, L.list_cap);
     EM_HIP(hipGetLastError());
     return EM_OK;
 }
