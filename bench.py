@@ -293,8 +293,9 @@ def extras(w, charges, seeds, plain_value, launch, only=None):
                     "lines at once in two pipelined groups; lines come from a counter until it "
                     "runs out, then every workgroup's lines are settled for up to 128 steps, "
                     "then the lines still moving are dealt out by work left and traced to the "
-                    "end; rows after a line's end are then filled. Best of 3 calls (kernel "
-                    "sequence event-timed)"}
+                    "end, while that last pass's idle waves write the rows after the end of "
+                    "every line that ended before it. Best of 3 calls (kernel sequence "
+                    "event-timed)"}
         del rows
     # 1b. lines_8k (1,000 charges in [-10,10]^3, 8,192 lines, 2,500 rows) in full:
     #     the GPU kernel (ref rule, resolution 10: the reference's live default
