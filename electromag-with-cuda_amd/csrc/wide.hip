@@ -31,9 +31,11 @@
 //             the lines still moving are, but for a few, the ones that never
 //             end, and each one's work left is known.
 //   FINISH -- list 2, ordered by steps taken (fewest first: most work left)
-//             and dealt in snake order, to the end.
+//             and dealt in snake order, to the end.  Its idle waves also
+//             write the rows after the end of every line that ended before
+//             it; k_wide_fill_list then writes those of list 2's lines.
 // k_wide_order sorts a list's entries by their step (one workgroup, bitonic
-// in LDS); k_ends_fill (ends.hip) then writes the rows after each line's end.
+// in LDS).
 #include <cstdlib>
 
 #include "em_internal.hpp"
