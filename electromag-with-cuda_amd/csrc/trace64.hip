@@ -17,13 +17,24 @@
 // and adds them in the same order; 256 lanes per workgroup; charges staged through
 // LDS as 32-byte records, all of them once when they fit a tile, else tile by
 // tile each step.  The charge loop is unrolled so the independent per-charge
-// terms (a square root and three divisions each) overlap while the additions
-// stay in order.
+// terms (a square root and one or three divisions each) overlap while the
+// additions stay in order.
+//
+// Square root and division: the compiler expands IEEE sqrt and division into
+// v_rsq_f64 / v_rcp_f64 plus FMA refinement, wrapped in range scaling and
+// special-case steps (the square root's ldexp scaling and class test;
+// v_div_scale, the scaling in v_div_fmas and v_div_fixup).  For operands in the
+// fast domain below those wrapper steps change no value, so the refinement
+// alone (sqrt_core, recip_core + div_core) gives the same correctly rounded
+// bits, and one refined reciprocal of |r|^2 sqrt(|r|^2) serves a term's three
+// divisions.  Tiles and points outside the domain take the compiler's
+// expansion (EMTRACE_FP64_EXACT=1 forces it everywhere).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "em_internal.hpp"
 
@@ -33,6 +44,7 @@ struct StepP64 {
     int rule;
     double res, h0, hmin, hmax, turn;
     int stop;
+    int exact;  // 1: the compiler's sqrt and division expansions for every term
 };
 
 namespace {
@@ -41,44 +53,200 @@ constexpr int kBlock64 = 256;
 constexpr uint32_t kTile64 = 2048;  // 64 KiB of LDS: two workgroups per CU
 constexpr double kCoulomb64 = 8.987551787E9;  // include/Electrostatics.h:27
 
+// The fast domain.  Every coordinate of the charges and of the current point is
+// 0 or has a magnitude in [2^-100, 2^190], and every charge magnitude |q| is in
+// [2^-100, 2^100].  Then each nonzero component of r = p - c is a nonzero
+// multiple of 2^-152 (both operands are multiples of it), so
+//   2^-304 <= |r|^2 <= 2^384 when r != 0,   |r|^2 sqrt(|r|^2) in [2^-456, 2^576],
+// and every numerator (k_e q, or r_i k q with k = k_e or 1; k_e < 2^34) is 0
+// or has a magnitude in [2^-252, 2^324], its quotient in [2^-828, 2^590]: no
+// operand or result is denormal, near overflow, or far enough apart in
+// exponent (768) for the wrapper steps to scale.  A zero numerator gives a
+// zero term whose sign may differ from the IEEE quotient's; added to a sum
+// that starts at +0 it changes no bit.  r = 0 (a point exactly on a charge)
+// gives NaN terms in both forms.
+constexpr double kDomLo = 0x1p-100, kDomHiX = 0x1p190, kDomHiQ = 0x1p100;
+
+__device__ __forceinline__ bool coord_in_domain(double v)
+{
+    const double a = fabs(v);
+    return v == 0.0 || (a >= kDomLo && a <= kDomHiX);
+}
+
+__device__ __forceinline__ bool charge_in_domain(const double4 c)
+{
+    const double a = fabs(c.w);
+    return coord_in_domain(c.x) && coord_in_domain(c.y) && coord_in_domain(c.z) &&
+           a >= kDomLo && a <= kDomHiQ;
+}
+
+// The charge record as staged in LDS: the plain rule's numerator k_e * q is the
+// same product for every line, formed once here.
 template <int RULE>
+__device__ __forceinline__ double4 staged(const double4 c)
+{
+    return RULE == kRulePlain ? make_double4(c.x, c.y, c.z, kCoulomb64 * c.w) : c;
+}
+
+// sqrt(x) for x in [2^-767, inf): the compiler's refinement of v_rsq_f64
+// without its scaling and class test (no-ops there).
+__device__ __forceinline__ double sqrt_core(double x)
+{
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double g = x * y0;
+    double h = y0 * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
+// The reciprocal refinement of the compiler's division (two Newton steps from
+// v_rcp_f64) and its final correction: div_core(n, d, recip_core(d)) == n / d
+// whenever v_div_scale and v_div_fixup would leave the operands and the
+// quotient unchanged.
+__device__ __forceinline__ double recip_core(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double div_core(double n, double d, double r)
+{
+    const double y = n * r;
+    const double e = __builtin_fma(-d, y, n);
+    return __builtin_fma(e, r, y);
+}
+
+// One PartField term; c.w as staged (k_e q for the plain rule, q otherwise).
+template <int RULE, bool FAST>
 __device__ __forceinline__ void term64(const double4 c, double px, double py, double pz,
                                        double& ex, double& ey, double& ez)
 {
     const double rx = px - c.x, ry = py - c.y, rz = pz - c.z;
     const double r2 = rx * rx + ry * ry + rz * rz;
-    const double den = r2 * sqrt(r2);
-    if (RULE == kRulePlain) {
-        const double s = kCoulomb64 * c.w / den;
-        ex = rx * s;
-        ey = ry * s;
-        ez = rz * s;
+    const double k = RULE == kRuleRef ? kCoulomb64 : 1.0;
+    if constexpr (FAST) {
+        const double den = r2 * sqrt_core(r2);
+        const double rd = recip_core(den);
+        if (RULE == kRulePlain) {
+            const double s = div_core(c.w, den, rd);
+            ex = rx * s;
+            ey = ry * s;
+            ez = rz * s;
+        } else {
+            ex = div_core(rx * k * c.w, den, rd);
+            ey = div_core(ry * k * c.w, den, rd);
+            ez = div_core(rz * k * c.w, den, rd);
+        }
     } else {
-        const double k = RULE == kRuleRef ? kCoulomb64 : 1.0;
-        ex = rx * k * c.w / den;
-        ey = ry * k * c.w / den;
-        ez = rz * k * c.w / den;
+        const double den = r2 * sqrt(r2);
+        if (RULE == kRulePlain) {
+            const double s = c.w / den;
+            ex = rx * s;
+            ey = ry * s;
+            ez = rz * s;
+        } else {
+            ex = rx * k * c.w / den;
+            ey = ry * k * c.w / den;
+            ez = rz * k * c.w / den;
+        }
     }
 }
 
-// Charges whose terms are in flight together.  With one wave per SIMD (65,536
-// lines) 4 is fastest: the plain rule's 250-row trace of 16,384 charges takes
-// 440 ms, against 489 ms at 2, 497 ms at 8 and 564 ms without unrolling.
-constexpr int kUnroll64 = 4;
+// U terms at once.  The fast form is written stage by stage across the U
+// terms (each a dependent chain of ~30 operations) so that their operations
+// interleave: with one wave per SIMD nothing else hides a chain's latency.
+// sched_barrier(0) after each stage keeps the compiler's scheduler from
+// regrouping the stages chain by chain.  Every term's own operations are those
+// of term64, in the same order.
+#define EM_STAGE(body)                               \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) { \
+        body;                                        \
+    }                                                \
+    __builtin_amdgcn_sched_barrier(0)
 
-template <int RULE>
+template <int RULE, bool FAST, int U>
+__device__ __forceinline__ void terms64(const double4 (&c)[U], double px, double py, double pz,
+                                        double (&e)[U][3])
+{
+    if constexpr (!FAST) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) term64<RULE, false>(c[u], px, py, pz, e[u][0], e[u][1], e[u][2]);
+    } else {
+        const double k = RULE == kRuleRef ? kCoulomb64 : 1.0;
+        double r2[U], g[U], h[U], t[U], d[U];
+        EM_STAGE(e[u][0] = px - c[u].x; e[u][1] = py - c[u].y; e[u][2] = pz - c[u].z);
+        EM_STAGE(r2[u] = e[u][0] * e[u][0] + e[u][1] * e[u][1] + e[u][2] * e[u][2]);
+        // sqrt_core
+        EM_STAGE(h[u] = __builtin_amdgcn_rsq(r2[u]));
+        EM_STAGE(g[u] = r2[u] * h[u]; h[u] = h[u] * 0.5);
+        EM_STAGE(t[u] = __builtin_fma(-h[u], g[u], 0.5));
+        EM_STAGE(g[u] = __builtin_fma(g[u], t[u], g[u]); h[u] = __builtin_fma(h[u], t[u], h[u]));
+        EM_STAGE(t[u] = __builtin_fma(-g[u], g[u], r2[u]));
+        EM_STAGE(g[u] = __builtin_fma(t[u], h[u], g[u]));
+        EM_STAGE(t[u] = __builtin_fma(-g[u], g[u], r2[u]));
+        EM_STAGE(g[u] = __builtin_fma(t[u], h[u], g[u]));
+        // den = |r|^2 sqrt(|r|^2) and recip_core(den)
+        EM_STAGE(d[u] = r2[u] * g[u]);
+        EM_STAGE(h[u] = __builtin_amdgcn_rcp(d[u]));
+        EM_STAGE(t[u] = __builtin_fma(-d[u], h[u], 1.0));
+        EM_STAGE(h[u] = __builtin_fma(h[u], t[u], h[u]));
+        EM_STAGE(t[u] = __builtin_fma(-d[u], h[u], 1.0));
+        EM_STAGE(h[u] = __builtin_fma(h[u], t[u], h[u]));
+        if constexpr (RULE == kRulePlain) {
+            // s = div_core(k_e q, den, rd), then r * s
+            EM_STAGE(g[u] = c[u].w * h[u]);
+            EM_STAGE(t[u] = __builtin_fma(-d[u], g[u], c[u].w));
+            EM_STAGE(g[u] = __builtin_fma(t[u], h[u], g[u]));
+            EM_STAGE(e[u][0] = e[u][0] * g[u]; e[u][1] = e[u][1] * g[u]; e[u][2] = e[u][2] * g[u]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                // e_j = div_core(r_j k q, den, rd)
+                EM_STAGE(r2[u] = e[u][j] * k * c[u].w);
+                EM_STAGE(g[u] = r2[u] * h[u]);
+                EM_STAGE(t[u] = __builtin_fma(-d[u], g[u], r2[u]));
+                EM_STAGE(e[u][j] = __builtin_fma(t[u], h[u], g[u]));
+            }
+        }
+    }
+}
+#undef EM_STAGE
+
+// Charges whose terms are in flight together.  The compiler's expansions
+// (kUnroll64): with one wave per SIMD (65,536 lines) 4 is fastest, the plain
+// rule's 250-row trace of 16,384 charges taking 440 ms, against 489 ms at 2,
+// 497 ms at 8 and 564 ms without unrolling.  The fast form (kUnroll64Fast,
+// stages interleaved): 321 ms at 8, against 359 ms at 4, 351 ms at 12 and
+// 324 ms at 16 (156 VGPRs at 8).
+constexpr int kUnroll64 = 4;
+#ifndef EMTRACE_FP64_UNROLL_FAST
+#define EMTRACE_FP64_UNROLL_FAST 8
+#endif
+constexpr int kUnroll64Fast = EMTRACE_FP64_UNROLL_FAST;
+
+template <int RULE, bool FAST>
 __device__ __forceinline__ void sum_range(const double4* __restrict__ tile, uint32_t count,
                                           double px, double py, double pz, double& sx, double& sy,
                                           double& sz)
 {
+    constexpr int U = FAST ? kUnroll64Fast : kUnroll64;
     uint32_t i = 0;
-    for (; i + kUnroll64 <= count; i += kUnroll64) {
-        double e[kUnroll64][3];
+    for (; i + U <= count; i += U) {
+        double4 c[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll64; ++u)
-            term64<RULE>(tile[i + u], px, py, pz, e[u][0], e[u][1], e[u][2]);
+        for (int u = 0; u < U; ++u) c[u] = tile[i + u];
+        double e[U][3];
+        terms64<RULE, FAST, U>(c, px, py, pz, e);
 #pragma unroll
-        for (int u = 0; u < kUnroll64; ++u) {
+        for (int u = 0; u < U; ++u) {
             sx += e[u][0];
             sy += e[u][1];
             sz += e[u][2];
@@ -86,7 +254,7 @@ __device__ __forceinline__ void sum_range(const double4* __restrict__ tile, uint
     }
     for (; i < count; ++i) {
         double ex, ey, ez;
-        term64<RULE>(tile[i], px, py, pz, ex, ey, ez);
+        term64<RULE, FAST>(tile[i], px, py, pz, ex, ey, ez);
         sx += ex;
         sy += ey;
         sz += ez;
@@ -127,23 +295,25 @@ __device__ __forceinline__ void add_in_order(double& s, double v, uint32_t m)
     }
 }
 
-template <int RULE, int S>
+template <int RULE, int S, bool FAST>
 __device__ __forceinline__ void sum_range_group(const double4* __restrict__ tile, uint32_t count,
                                                 uint32_t k, double px, double py, double pz,
                                                 double& sx, double& sy, double& sz)
 {
     if constexpr (S == 1) {
-        sum_range<RULE>(tile, count, px, py, pz, sx, sy, sz);
+        sum_range<RULE, FAST>(tile, count, px, py, pz, sx, sy, sz);
     } else {
-        constexpr uint32_t G = S * kUnroll64;  // charges per pass
+        constexpr int U = FAST ? kUnroll64Fast : kUnroll64;
+        constexpr uint32_t G = S * U;  // charges per pass
         uint32_t i = 0;
         for (; i + G <= count; i += G) {
-            double e[kUnroll64][3];
+            double4 c[U];
 #pragma unroll
-            for (int u = 0; u < kUnroll64; ++u)
-                term64<RULE>(tile[i + u * S + k], px, py, pz, e[u][0], e[u][1], e[u][2]);
+            for (int u = 0; u < U; ++u) c[u] = tile[i + u * S + k];
+            double e[U][3];
+            terms64<RULE, FAST, U>(c, px, py, pz, e);
 #pragma unroll
-            for (int u = 0; u < kUnroll64; ++u) {
+            for (int u = 0; u < U; ++u) {
                 add_in_order<S>(sx, e[u][0], S);
                 add_in_order<S>(sy, e[u][1], S);
                 add_in_order<S>(sz, e[u][2], S);
@@ -154,7 +324,7 @@ __device__ __forceinline__ void sum_range_group(const double4* __restrict__ tile
             // (uniform over the workgroup: every lane sees the same count)
             const uint32_t m = min((uint32_t)S, count - i);
             double ex = 0.0, ey = 0.0, ez = 0.0;
-            if (k < m) term64<RULE>(tile[i + k], px, py, pz, ex, ey, ez);
+            if (k < m) term64<RULE, FAST>(tile[i + k], px, py, pz, ex, ey, ez);
             add_in_order<S>(sx, ex, m);
             add_in_order<S>(sy, ey, m);
             add_in_order<S>(sz, ez, m);
@@ -249,24 +419,50 @@ __global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict_
     st.steps = row0 - 1;
     st.alive = valid;
     const bool single = p <= tile;
+    // the tile holds a charge outside the fast domain (staged tiles: every step)
+    bool tile_out = sp.exact != 0;
     if (single) {
-        for (uint32_t i = threadIdx.x; i < p; i += kBlock64) lds64[i] = charges[i];
-        __syncthreads();
+        bool out = false;
+        for (uint32_t i = threadIdx.x; i < p; i += kBlock64) {
+            const double4 c = charges[i];
+            out |= !charge_in_domain(c);
+            lds64[i] = staged<RULE>(c);
+        }
+        tile_out = __syncthreads_or(out) || tile_out;
     }
     for (uint32_t s = row0; s < len; ++s) {
         // the workgroup skips the field once none of its lines is alive
         if (__syncthreads_or(st.alive)) {
             double ex = 0.0, ey = 0.0, ez = 0.0;
+            // wave-uniform: every moving line of the wave has its point in the domain
+            const bool pts = __all(!st.alive || (coord_in_domain(st.px) && coord_in_domain(st.py) &&
+                                                  coord_in_domain(st.pz)));
             if (single) {
-                if (st.alive) sum_range_group<RULE, S>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez);
+                if (st.alive) {
+                    if (pts && !tile_out)
+                        sum_range_group<RULE, S, true>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez);
+                    else
+                        sum_range_group<RULE, S, false>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez);
+                }
             } else {
                 for (uint32_t t0 = 0; t0 < p; t0 += tile) {
                     const uint32_t cnt = min(tile, p - t0);
                     __syncthreads();
-                    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock64) lds64[i] = charges[t0 + i];
-                    __syncthreads();
-                    if (st.alive)
-                        sum_range_group<RULE, S>(lds64, cnt, k, st.px, st.py, st.pz, ex, ey, ez);
+                    bool out = false;
+                    for (uint32_t i = threadIdx.x; i < cnt; i += kBlock64) {
+                        const double4 c = charges[t0 + i];
+                        out |= !charge_in_domain(c);
+                        lds64[i] = staged<RULE>(c);
+                    }
+                    const bool fast = !__syncthreads_or(out) && pts && sp.exact == 0;
+                    if (st.alive) {
+                        if (fast)
+                            sum_range_group<RULE, S, true>(lds64, cnt, k, st.px, st.py, st.pz, ex,
+                                                           ey, ez);
+                        else
+                            sum_range_group<RULE, S, false>(lds64, cnt, k, st.px, st.py, st.pz, ex,
+                                                            ey, ez);
+                    }
                 }
             }
             if (st.alive) step64<RULE>(sp, s, ex, ey, ez, st);
@@ -313,6 +509,7 @@ StepP64 from_step(const em_step& st)
     sp.hmax = st.hmax;
     sp.turn = st.turn;
     sp.stop = st.stop ? 1 : 0;
+    sp.exact = 0;
     return sp;
 }
 
@@ -363,15 +560,20 @@ int launch64(const DeviceInfo& dev, const double* charges, uint32_t p, double* x
     const size_t lds = (size_t)tile * sizeof(double4);
     const uint32_t S = lanes64(dev, n);
     const double4* q = reinterpret_cast<const double4*>(charges);
+    // EMTRACE_FP64_EXACT=1: every term through the compiler's sqrt and division
+    // expansions (the same bits; for tests and timing of the fast domain)
+    const char* env = std::getenv("EMTRACE_FP64_EXACT");
+    StepP64 spk = sp;
+    spk.exact = env && *env && *env != '0' ? 1 : 0;
     switch (sp.rule) {
     case kRulePlain:
-        launch64_rule<kRulePlain>(S, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+        launch64_rule<kRulePlain>(S, lds, stream, q, p, x, y, z, n, len, pitch, spk, tile, steps, row0);
         break;
     case kRuleRef:
-        launch64_rule<kRuleRef>(S, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+        launch64_rule<kRuleRef>(S, lds, stream, q, p, x, y, z, n, len, pitch, spk, tile, steps, row0);
         break;
     default:
-        launch64_rule<kRuleAngle>(S, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+        launch64_rule<kRuleAngle>(S, lds, stream, q, p, x, y, z, n, len, pitch, spk, tile, steps, row0);
         break;
     }
     EM_HIP(hipGetLastError());

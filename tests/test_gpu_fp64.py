@@ -4,6 +4,9 @@
   angle with and without line ends, single- and multi-tile charge staging,
   partial workgroups, one, two and four lanes per line, pitch > n on the
   device form;
+* the square roots and divisions that skip the compiler's range wrappers
+  inside the fast domain give the same bits as the compiler's expansions
+  (all of lines_64k) and as the restatement at the domain's edges;
 * the host-array form traces the plain and ref rules in row chunks, bit-equal
   to one device launch;
 * against the reference's own double rows (golden vectors) it is as close as
@@ -69,6 +72,80 @@ def test_fp64_bitwise_equals_restatement(gpu, orc, case):
           f"{np.mean(steps.cpu().numpy() == want_steps.astype(np.int64)):.4f}", flush=True)
     assert same.all()
     assert np.array_equal(steps.cpu().numpy(), want_steps.astype(np.int32))
+
+
+def _edge_problem(kind, p, n, length, seed):
+    """Problems at the edges of the kernel's fast square-root and division
+    domain (coordinates 0 or of magnitude 2^-100 .. 2^190, |q| 2^-100 .. 2^100)."""
+    w = gen.plain_config(p, n, 5.0, length, seed=seed)
+    ch, sd = w.charges.astype(np.float64), w.seeds.astype(np.float64)
+    if kind == "planar":  # r_z = 0: zero numerators inside the domain
+        ch[:, 2] = 0.0
+        sd[:, 2] = 0.0
+    elif kind == "tiny_coordinate":  # 1e-35 < 2^-100: that charge's tile leaves the domain
+        ch[p // 2, 0] = 1e-35
+    elif kind == "zero_charge":
+        ch[p // 3, 3] = 0.0
+    elif kind == "far_charge":  # 1e60 > 2^190
+        ch[1, :3] = (1e60, -2e60, 3e60)
+    elif kind == "tiny_point":  # one line's first point leaves the domain
+        sd[5, 1] = 1e-40
+    else:
+        raise ValueError(kind)
+    return ch, sd
+
+
+EDGES = [(kind, p, n, length) for kind in ("planar", "tiny_coordinate", "zero_charge", "far_charge",
+                                           "tiny_point")
+         for (p, n, length) in ((300, 512, 60), (3000, 128, 12))] + [
+    # one lane per line (65,536 lines)
+    ("planar", 40, 65536, 8), ("tiny_coordinate", 40, 65536, 8)]
+
+
+@pytest.mark.parametrize("edge", EDGES, ids=[f"{e[0]}-p{e[1]}-n{e[2]}" for e in EDGES])
+@pytest.mark.parametrize("rule", [rules.plain(10.0), rules.ref(10.0), rules.angle()],
+                         ids=["plain", "ref", "angle"])
+def test_fp64_fast_domain_edges_bitwise(gpu, orc, edge, rule):
+    """Inside the fast domain the kernel's square roots and divisions skip the
+    compiler's range wrappers; outside it (a tile with a charge beyond it, a
+    wave with a point beyond it) the compiler's expansions run.  Both give the
+    restatement's bits."""
+    kind, p, n, length = edge
+    rule = _as_f32_params(rule)
+    ch, sd = _edge_problem(kind, p, n, length, seed=19)
+    rc, want, want_steps = orc.calc_lines(ch, sd, length, rule, dtype=np.float64, with_steps=True)
+    assert rc == 0
+    rows, steps = gpu.trace64(torch.from_numpy(ch).cuda(), torch.from_numpy(sd).cuda(), length, rule,
+                              with_steps=True)
+    rows = rows.cpu().numpy()
+    same = (rows == want) | (np.isnan(rows) & np.isnan(want))
+    print(f"[fp64 edge {kind} p={p} n={n} {rules.NAMES[rule.rule]}] identical {same.mean():.6f}",
+          flush=True)
+    assert same.all()
+    assert np.array_equal(steps.cpu().numpy(), want_steps.astype(np.int32))
+
+
+@pytest.mark.parametrize("rule", [rules.plain(10.0), rules.ref(10.0), rules.angle()],
+                         ids=["plain", "ref", "angle"])
+def test_fp64_fast_form_equals_compiler_expansions_full_lines_64k(gpu, monkeypatch, rule):
+    """All 16,384 lines_64k charges and 65,536 lines (one lane per line, staged
+    tiles), 40 rows: the fast square roots and divisions against the compiler's
+    expansions (EMTRACE_FP64_EXACT=1), bit for bit over 4.2e10 terms."""
+    w = gen.lines_64k()
+    length = 40
+    rule = _as_f32_params(rule)
+    q = torch.from_numpy(w.charges.astype(np.float64)).cuda()
+    s = torch.from_numpy(w.seeds.astype(np.float64)).cuda()
+    fast, fs = gpu.trace64(q, s, length, rule, with_steps=True)
+    monkeypatch.setenv("EMTRACE_FP64_EXACT", "1")
+    exact, es = gpu.trace64(q, s, length, rule, with_steps=True)
+    monkeypatch.delenv("EMTRACE_FP64_EXACT")
+    fast, exact = fast.cpu().numpy(), exact.cpu().numpy()
+    same = (fast == exact) | (np.isnan(fast) & np.isnan(exact))
+    print(f"[fp64 fast vs exact lines_64k {rules.NAMES[rule.rule]}] identical {same.mean():.8f}",
+          flush=True)
+    assert np.isfinite(exact).all()
+    assert same.all() and torch.equal(fs, es)
 
 
 def test_fp64_host_and_pitched_device_forms(gpu, orc):
