@@ -166,15 +166,29 @@ __device__ __forceinline__ void term64(const double4 c, double px, double py, do
 // sched_barrier(0) after each stage keeps the compiler's scheduler from
 // regrouping the stages chain by chain.  Every term's own operations are those
 // of term64, in the same order.
-#define EM_STAGE(body)                               \
+// Work threaded through the stages of terms64: after stage I, side.at<I>().
+struct NoSide {
+    template <int I>
+    __device__ __forceinline__ void at() {}
+};
+
+#define EM_STAGE(I, body)                            \
     _Pragma("unroll") for (int u = 0; u < U; ++u) { \
         body;                                        \
     }                                                \
+    side.template at<I>();                           \
     __builtin_amdgcn_sched_barrier(0)
 
-template <int RULE, bool FAST, int U>
+// the division of component J: e_J = div_core(r_J k q, den, rd), stages I..I+3
+#define EM_DIV_STAGES(I, J)                                               \
+    EM_STAGE(I, r2[u] = e[u][J] * k * c[u].w);                            \
+    EM_STAGE(I + 1, g[u] = r2[u] * h[u]);                                 \
+    EM_STAGE(I + 2, t[u] = __builtin_fma(-d[u], g[u], r2[u]));            \
+    EM_STAGE(I + 3, e[u][J] = __builtin_fma(t[u], h[u], g[u]))
+
+template <int RULE, bool FAST, int U, class Side = NoSide>
 __device__ __forceinline__ void terms64(const double4 (&c)[U], double px, double py, double pz,
-                                        double (&e)[U][3])
+                                        double (&e)[U][3], Side side = Side())
 {
     if constexpr (!FAST) {
 #pragma unroll
@@ -182,42 +196,38 @@ __device__ __forceinline__ void terms64(const double4 (&c)[U], double px, double
     } else {
         const double k = RULE == kRuleRef ? kCoulomb64 : 1.0;
         double r2[U], g[U], h[U], t[U], d[U];
-        EM_STAGE(e[u][0] = px - c[u].x; e[u][1] = py - c[u].y; e[u][2] = pz - c[u].z);
-        EM_STAGE(r2[u] = e[u][0] * e[u][0] + e[u][1] * e[u][1] + e[u][2] * e[u][2]);
+        EM_STAGE(0, e[u][0] = px - c[u].x; e[u][1] = py - c[u].y; e[u][2] = pz - c[u].z);
+        EM_STAGE(1, r2[u] = e[u][0] * e[u][0] + e[u][1] * e[u][1] + e[u][2] * e[u][2]);
         // sqrt_core
-        EM_STAGE(h[u] = __builtin_amdgcn_rsq(r2[u]));
-        EM_STAGE(g[u] = r2[u] * h[u]; h[u] = h[u] * 0.5);
-        EM_STAGE(t[u] = __builtin_fma(-h[u], g[u], 0.5));
-        EM_STAGE(g[u] = __builtin_fma(g[u], t[u], g[u]); h[u] = __builtin_fma(h[u], t[u], h[u]));
-        EM_STAGE(t[u] = __builtin_fma(-g[u], g[u], r2[u]));
-        EM_STAGE(g[u] = __builtin_fma(t[u], h[u], g[u]));
-        EM_STAGE(t[u] = __builtin_fma(-g[u], g[u], r2[u]));
-        EM_STAGE(g[u] = __builtin_fma(t[u], h[u], g[u]));
+        EM_STAGE(2, h[u] = __builtin_amdgcn_rsq(r2[u]));
+        EM_STAGE(3, g[u] = r2[u] * h[u]; h[u] = h[u] * 0.5);
+        EM_STAGE(4, t[u] = __builtin_fma(-h[u], g[u], 0.5));
+        EM_STAGE(5, g[u] = __builtin_fma(g[u], t[u], g[u]); h[u] = __builtin_fma(h[u], t[u], h[u]));
+        EM_STAGE(6, t[u] = __builtin_fma(-g[u], g[u], r2[u]));
+        EM_STAGE(7, g[u] = __builtin_fma(t[u], h[u], g[u]));
+        EM_STAGE(8, t[u] = __builtin_fma(-g[u], g[u], r2[u]));
+        EM_STAGE(9, g[u] = __builtin_fma(t[u], h[u], g[u]));
         // den = |r|^2 sqrt(|r|^2) and recip_core(den)
-        EM_STAGE(d[u] = r2[u] * g[u]);
-        EM_STAGE(h[u] = __builtin_amdgcn_rcp(d[u]));
-        EM_STAGE(t[u] = __builtin_fma(-d[u], h[u], 1.0));
-        EM_STAGE(h[u] = __builtin_fma(h[u], t[u], h[u]));
-        EM_STAGE(t[u] = __builtin_fma(-d[u], h[u], 1.0));
-        EM_STAGE(h[u] = __builtin_fma(h[u], t[u], h[u]));
+        EM_STAGE(10, d[u] = r2[u] * g[u]);
+        EM_STAGE(11, h[u] = __builtin_amdgcn_rcp(d[u]));
+        EM_STAGE(12, t[u] = __builtin_fma(-d[u], h[u], 1.0));
+        EM_STAGE(13, h[u] = __builtin_fma(h[u], t[u], h[u]));
+        EM_STAGE(14, t[u] = __builtin_fma(-d[u], h[u], 1.0));
+        EM_STAGE(15, h[u] = __builtin_fma(h[u], t[u], h[u]));
         if constexpr (RULE == kRulePlain) {
             // s = div_core(k_e q, den, rd), then r * s
-            EM_STAGE(g[u] = c[u].w * h[u]);
-            EM_STAGE(t[u] = __builtin_fma(-d[u], g[u], c[u].w));
-            EM_STAGE(g[u] = __builtin_fma(t[u], h[u], g[u]));
-            EM_STAGE(e[u][0] = e[u][0] * g[u]; e[u][1] = e[u][1] * g[u]; e[u][2] = e[u][2] * g[u]);
+            EM_STAGE(16, g[u] = c[u].w * h[u]);
+            EM_STAGE(17, t[u] = __builtin_fma(-d[u], g[u], c[u].w));
+            EM_STAGE(18, g[u] = __builtin_fma(t[u], h[u], g[u]));
+            EM_STAGE(19, e[u][0] = e[u][0] * g[u]; e[u][1] = e[u][1] * g[u]; e[u][2] = e[u][2] * g[u]);
         } else {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                // e_j = div_core(r_j k q, den, rd)
-                EM_STAGE(r2[u] = e[u][j] * k * c[u].w);
-                EM_STAGE(g[u] = r2[u] * h[u]);
-                EM_STAGE(t[u] = __builtin_fma(-d[u], g[u], r2[u]));
-                EM_STAGE(e[u][j] = __builtin_fma(t[u], h[u], g[u]));
-            }
+            EM_DIV_STAGES(16, 0);
+            EM_DIV_STAGES(20, 1);
+            EM_DIV_STAGES(24, 2);
         }
     }
 }
+#undef EM_DIV_STAGES
 #undef EM_STAGE
 
 // Charges whose terms are in flight together.  The compiler's expansions
@@ -295,10 +305,124 @@ __device__ __forceinline__ void add_in_order(double& s, double v, uint32_t m)
     }
 }
 
+// Four-lane groups in the fast form: lane k of a group adds only component k
+// (lane 3 repeats component 2), so each lane adds 16 terms per pass instead of
+// reading all 48 term components through DPP moves.  The group's terms pass
+// through a per-wave LDS buffer: written at the end of a pass, read at the
+// start of the next, before that pass overwrites them (LDS operations of one
+// wave complete in program order, so a lane reads what the other lanes of its
+// group wrote one pass earlier).  The additions run in charge order, as in the
+// one-lane loop, threaded through the next pass's term stages; the three
+// totals are broadcast back to the group before the last charges of the tile.
+//
+// Buffer layout, in 16-byte chunks of two doubles: chunk (q, g, c) at
+// q * kSplitQ + 3 g + c holds elements 2q, 2q+1 of component c of group g,
+// where element j U + u is lane j's term u.  A reader (g, c) takes chunks
+// q = 0 .. 2U-1 in turn; a quarter-wave's 12 distinct chunks are consecutive,
+// so at most two share a bank.  kSplitQ = 49 (not 48) staggers the writers'
+// chunks (lane k writes q = 2k, 2k+1 for U = 4) across banks as well.
+constexpr int kSplitU = 4;                                      // terms per lane per pass
+constexpr uint32_t kSplitQ = 16 * 3 + 1;                         // chunks per q
+constexpr uint32_t kSplitBuf = 2 * kSplitU * kSplitQ * 2;        // doubles per wave
+constexpr uint32_t kSplitLdsBytes = (kBlock64 / 64) * kSplitBuf * sizeof(double);  // 24.5 KiB
+constexpr uint32_t kTileSplit = 1024;  // records staged at once: 32 KiB + 24.5 KiB, two per CU
+
+// The previous pass's additions, threaded through this pass's term stages
+// (two per stage from stage 5 on, by when the LDS reads have landed): element e
+// of the pass is lane e % 4's term u = e / 4, added in e order.
+struct SplitAdds {
+    double& acc;
+    const double (&v)[4 * kSplitU];
+    template <int I>
+    __device__ __forceinline__ void at()
+    {
+        constexpr int kFirst = 5, kPer = 2;
+        static_assert(kFirst + 4 * kSplitU / kPer <= 20, "the additions fit the plain rule's stages");
+        if constexpr (I >= kFirst) {
+#pragma unroll
+            for (int e = (I - kFirst) * kPer; e < (I - kFirst + 1) * kPer && e < 4 * kSplitU; ++e)
+                acc += v[(e % 4) * kSplitU + e / 4];
+        }
+    }
+};
+
+template <int RULE>
+__device__ __forceinline__ uint32_t sum_passes_split(const double4* __restrict__ tile,
+                                                     uint32_t count, uint32_t k, double px,
+                                                     double py, double pz, double& sx, double& sy,
+                                                     double& sz, double* __restrict__ xw)
+{
+    constexpr int S = 4, U = kSplitU;
+    constexpr uint32_t G = S * U;
+    const uint32_t g = (threadIdx.x % 64) / S;
+    const uint32_t comp = k < 3 ? k : 2;
+    double acc = k == 0 ? sx : (k == 1 ? sy : sz);
+    double2* const xc = reinterpret_cast<double2*>(xw);
+    const uint32_t passes = count / G;
+    auto write = [&](const double (&e)[U][3]) {
+        // lane k's term u is element k U + u: chunk (k U + u) / 2
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+            for (int u = 0; u < U; u += 2)
+                xc[((k * U + u) / 2) * kSplitQ + 3 * g + cc] = make_double2(e[u][cc], e[u + 1][cc]);
+        __builtin_amdgcn_wave_barrier();
+    };
+    auto read = [&](double (&v)[S * U]) {
+#pragma unroll
+        for (int q = 0; q < S * U / 2; ++q) {
+            const double2 t = xc[q * kSplitQ + 3 * g + comp];
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    };
+    // each pass's charge records are read one pass ahead
+    double4 c[U];
+    if (passes > 0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = tile[u * S + k];
+        const uint32_t nx = passes > 1 ? G : 0;
+        double4 cn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) cn[u] = tile[nx + u * S + k];
+        __builtin_amdgcn_sched_barrier(0);
+        double e[U][3];
+        terms64<RULE, true, U>(c, px, py, pz, e);
+        write(e);
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = cn[u];
+    }
+    for (uint32_t pass = 1; pass < passes; ++pass) {
+        double v[S * U];  // the previous pass's terms of component comp, [lane][u]
+        read(v);
+        const uint32_t nx = (pass + 1 < passes ? pass + 1 : pass) * G;
+        double4 cn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) cn[u] = tile[nx + u * S + k];
+        __builtin_amdgcn_sched_barrier(0);
+        double e[U][3];
+        terms64<RULE, true, U>(c, px, py, pz, e, SplitAdds{acc, v});
+        write(e);
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = cn[u];
+    }
+    if (passes > 0) {
+        double v[S * U];
+        read(v);
+#pragma unroll
+        for (int e = 0; e < S * U; ++e) acc += v[(e % S) * U + e / S];
+    }
+    sx = from_lane<S, 0>(acc);
+    sy = from_lane<S, 1>(acc);
+    sz = from_lane<S, 2>(acc);
+    return passes * G;
+}
+
 template <int RULE, int S, bool FAST>
 __device__ __forceinline__ void sum_range_group(const double4* __restrict__ tile, uint32_t count,
                                                 uint32_t k, double px, double py, double pz,
-                                                double& sx, double& sy, double& sz)
+                                                double& sx, double& sy, double& sz,
+                                                double* __restrict__ xw)
 {
     if constexpr (S == 1) {
         sum_range<RULE, FAST>(tile, count, px, py, pz, sx, sy, sz);
@@ -306,6 +430,7 @@ __device__ __forceinline__ void sum_range_group(const double4* __restrict__ tile
         constexpr int U = FAST ? kUnroll64Fast : kUnroll64;
         constexpr uint32_t G = S * U;  // charges per pass
         uint32_t i = 0;
+        if constexpr (S == 4 && FAST) i = sum_passes_split<RULE>(tile, count, k, px, py, pz, sx, sy, sz, xw);
         for (; i + G <= count; i += G) {
             double4 c[U];
 #pragma unroll
@@ -419,6 +544,9 @@ __global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict_
     st.steps = row0 - 1;
     st.alive = valid;
     const bool single = p <= tile;
+    // this wave's exchange buffers (four-lane groups), after the tile
+    double* const xw = S == 4 ? reinterpret_cast<double*>(lds64 + tile) + (threadIdx.x / 64) * kSplitBuf
+                              : nullptr;
     // the tile holds a charge outside the fast domain (staged tiles: every step)
     bool tile_out = sp.exact != 0;
     if (single) {
@@ -440,9 +568,9 @@ __global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict_
             if (single) {
                 if (st.alive) {
                     if (pts && !tile_out)
-                        sum_range_group<RULE, S, true>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez);
+                        sum_range_group<RULE, S, true>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez, xw);
                     else
-                        sum_range_group<RULE, S, false>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez);
+                        sum_range_group<RULE, S, false>(lds64, p, k, st.px, st.py, st.pz, ex, ey, ez, xw);
                 }
             } else {
                 for (uint32_t t0 = 0; t0 < p; t0 += tile) {
@@ -458,10 +586,10 @@ __global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict_
                     if (st.alive) {
                         if (fast)
                             sum_range_group<RULE, S, true>(lds64, cnt, k, st.px, st.py, st.pz, ex,
-                                                           ey, ez);
+                                                           ey, ez, xw);
                         else
                             sum_range_group<RULE, S, false>(lds64, cnt, k, st.px, st.py, st.pz, ex,
-                                                            ey, ez);
+                                                            ey, ez, xw);
                     }
                 }
             }
@@ -549,6 +677,15 @@ void launch64_rule(uint32_t S, size_t lds, hipStream_t stream, const double4* q,
         launch64_s<RULE, 1>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
 }
 
+// Charge records staged in LDS at once: all of them up to kTile64 (64 KiB, two
+// workgroups per CU), else tiles of half that; four-lane groups also hold their
+// exchange buffers (24.5 KiB), so their tiles stop at kTileSplit.
+uint32_t tile64(uint32_t p, uint32_t S)
+{
+    const uint32_t cap = S == 4 ? kTileSplit : kTile64;
+    return p <= cap ? std::max<uint32_t>(p, 1) : kTile64 / 2;
+}
+
 // Rows row0 .. len-1 (row0 > 1: plain and ref rules only).
 int launch64(const DeviceInfo& dev, const double* charges, uint32_t p, double* x, double* y,
              double* z, uint32_t n, uint32_t len, uint64_t pitch, const StepP64& sp,
@@ -556,9 +693,9 @@ int launch64(const DeviceInfo& dev, const double* charges, uint32_t p, double* x
 {
     if (row0 < 1 || (row0 > 1 && sp.rule == kRuleAngle))
         EM_FAIL(EM_ERR_INVALID, "row0 %u: the angle rule traces from row 1", row0);
-    const uint32_t tile = p <= kTile64 ? std::max<uint32_t>(p, 1) : kTile64 / 2;
-    const size_t lds = (size_t)tile * sizeof(double4);
     const uint32_t S = lanes64(dev, n);
+    const uint32_t tile = tile64(p, S);
+    const size_t lds = (size_t)tile * sizeof(double4) + (S == 4 ? kSplitLdsBytes : 0);
     const double4* q = reinterpret_cast<const double4*>(charges);
     // EMTRACE_FP64_EXACT=1: every term through the compiler's sqrt and division
     // expansions (the same bits; for tests and timing of the fast domain)
@@ -663,7 +800,7 @@ int host_trace64(const double* charges, size_t p, double* x, double* y, double* 
         // CPU_Implement.cpp:347-349
         perf->gflops = (dn * (dl - 1.0) * (dp * 18.0 + 13.0)) / perf->time_s / 1e9;
         perf->lanes_per_line = (int32_t)lanes64(dev, (uint32_t)n);
-        perf->tile = (int32_t)(p <= kTile64 ? std::max<size_t>(p, 1) : kTile64 / 2);
+        perf->tile = (int32_t)tile64((uint32_t)p, lanes64(dev, (uint32_t)n));
         perf->block = kBlock64;
         perf->unroll = 0;  // the FP64 kernel's loop has no pipelined groups
         set_progress(perf, 1.0);
