@@ -160,12 +160,6 @@ __device__ __forceinline__ void term64(const double4 c, double px, double py, do
     }
 }
 
-// U terms at once.  The fast form is written stage by stage across the U
-// terms (each a dependent chain of ~30 operations) so that their operations
-// interleave: with one wave per SIMD nothing else hides a chain's latency.
-// sched_barrier(0) after each stage keeps the compiler's scheduler from
-// regrouping the stages chain by chain.  Every term's own operations are those
-// of term64, in the same order.
 // Work threaded through the stages of terms64: after stage I, side.at<I>().
 struct NoSide {
     template <int I>
@@ -186,6 +180,12 @@ struct NoSide {
     EM_STAGE(I + 2, t[u] = __builtin_fma(-d[u], g[u], r2[u]));            \
     EM_STAGE(I + 3, e[u][J] = __builtin_fma(t[u], h[u], g[u]))
 
+// U terms at once.  The fast form is written stage by stage across the U
+// terms (each a dependent chain of ~30 operations) so that their operations
+// interleave: with one wave per SIMD nothing else hides a chain's latency.
+// sched_barrier(0) after each stage keeps the compiler's scheduler from
+// regrouping the stages chain by chain.  Every term's own operations are those
+// of term64, in the same order.
 template <int RULE, bool FAST, int U, class Side = NoSide>
 __device__ __forceinline__ void terms64(const double4 (&c)[U], double px, double py, double pz,
                                         double (&e)[U][3], Side side = Side())
