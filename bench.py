@@ -85,25 +85,37 @@ def timed(fn, steps, warmup):
 
 
 def host_cpu():
-    """The host CPU as the baseline runs on it: model name, CPUs this process
-    may run on (affinity), CPUs in the machine, and the OpenMP thread count a
-    parallel region gets (what the reference's OpenMP loop uses)."""
+    """The host CPU as the baseline runs on it: model name with its x86 family,
+    model and stepping numbers (a virtual machine may report only a family
+    name), the SIMD extensions the reference's SSE path could use, CPUs this
+    process may run on (affinity), CPUs in the machine, and the OpenMP thread
+    count a parallel region gets (what the reference's OpenMP loop uses)."""
     import ctypes
-    model = "unknown"
+    model, ident, flags = "unknown", {}, set()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model == "unknown":
+                    model = val
+                elif key in ("vendor_id", "cpu family", "model", "stepping") and key not in ident:
+                    ident[key] = val
+                elif key == "flags" and not flags:
+                    flags = set(val.split())
+                elif not line.strip() and model != "unknown":
+                    break  # the first processor's block is enough
     except OSError:
         pass
+    if ident:
+        model += " (" + ", ".join(f"{k} {v}" for k, v in ident.items()) + ")"
     try:
         omp = int(ctypes.CDLL("libgomp.so.1").omp_get_max_threads())
     except OSError:
         omp = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     affinity = len(os.sched_getaffinity(0))
-    info = {"cpu_model": model, "omp_threads": omp, "affinity_cpus": affinity,
+    simd = [f for f in ("sse2", "avx2", "avx512f") if f in flags]
+    info = {"cpu_model": model, "cpu_simd": simd, "omp_threads": omp, "affinity_cpus": affinity,
             "machine_cpus": os.cpu_count()}
     if omp < affinity:
         info["threads_note"] = (f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')} caps the "
