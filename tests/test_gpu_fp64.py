@@ -6,7 +6,9 @@
   device form;
 * the square roots and divisions that skip the compiler's range wrappers
   inside the fast domain give the same bits as the compiler's expansions
-  (all of lines_64k) and as the restatement at the domain's edges;
+  (all of lines_64k; four-lane groups, each lane adding one component, on the
+  reference's default problem and on 16,384 lines_64k lines) and as the
+  restatement at the domain's edges;
 * the host-array form traces the plain and ref rules in row chunks, bit-equal
   to one device launch;
 * against the reference's own double rows (golden vectors) it is as close as
@@ -136,16 +138,41 @@ def test_fp64_fast_form_equals_compiler_expansions_full_lines_64k(gpu, monkeypat
     rule = _as_f32_params(rule)
     q = torch.from_numpy(w.charges.astype(np.float64)).cuda()
     s = torch.from_numpy(w.seeds.astype(np.float64)).cuda()
+    _fast_vs_exact(gpu, monkeypatch, q, s, length, rule, f"lines_64k {rules.NAMES[rule.rule]}")
+
+
+def _fast_vs_exact(gpu, monkeypatch, q, s, length, rule, label):
     fast, fs = gpu.trace64(q, s, length, rule, with_steps=True)
     monkeypatch.setenv("EMTRACE_FP64_EXACT", "1")
     exact, es = gpu.trace64(q, s, length, rule, with_steps=True)
     monkeypatch.delenv("EMTRACE_FP64_EXACT")
     fast, exact = fast.cpu().numpy(), exact.cpu().numpy()
     same = (fast == exact) | (np.isnan(fast) & np.isnan(exact))
-    print(f"[fp64 fast vs exact lines_64k {rules.NAMES[rule.rule]}] identical {same.mean():.8f}",
-          flush=True)
+    print(f"[fp64 fast vs exact {label}] identical {same.mean():.8f}", flush=True)
     assert np.isfinite(exact).all()
     assert same.all() and torch.equal(fs, es)
+
+
+def test_fp64_four_lane_groups_equal_compiler_expansions_default_problem(gpu, monkeypatch):
+    """The reference application's default problem (1,024 charges, 16,384
+    lines, 2,500 rows, curvature step) on doubles: four lanes per line, one
+    staged tile, each lane adding one component of its group's terms; bit for
+    bit equal to the compiler's square roots and divisions with every lane
+    adding all terms (4.2e10 terms)."""
+    w = legacy.preset("default")
+    q = torch.from_numpy(w.charges.astype(np.float64)).cuda()
+    s = torch.from_numpy(w.seeds.astype(np.float64)).cuda()
+    _fast_vs_exact(gpu, monkeypatch, q, s, w.len, _as_f32_params(rules.ref(1.0)), "default problem")
+
+
+@pytest.mark.parametrize("rule", [rules.plain(10.0), rules.angle()], ids=["plain", "angle"])
+def test_fp64_four_lane_groups_staged_tiles_equal_compiler_expansions(gpu, monkeypatch, rule):
+    """All 16,384 lines_64k charges (16 staged tiles per step) and every fourth
+    of its lines (16,384: four lanes per line), 60 rows."""
+    w = gen.lines_64k()
+    q = torch.from_numpy(w.charges.astype(np.float64)).cuda()
+    s = torch.from_numpy(np.ascontiguousarray(w.seeds[::4]).astype(np.float64)).cuda()
+    _fast_vs_exact(gpu, monkeypatch, q, s, 60, _as_f32_params(rule), "lines_64k / 4")
 
 
 def test_fp64_host_and_pitched_device_forms(gpu, orc):
