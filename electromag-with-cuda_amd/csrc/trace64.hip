@@ -641,16 +641,24 @@ StepP64 from_step(const em_step& st)
     return sp;
 }
 
-// Rows row0 .. len-1 (row0 > 1: plain and ref rules only).
-// Lanes per line.  One lane per line gives every SIMD a wave (256 CUs x 4
-// SIMDs x 64 lanes = 65,536 lanes) only from 65,536 lines on: fewer lines take
-// S = 2 lanes each, and at most half that many, S = 4.
+// Lanes per line.  A wave's time is set by its lanes' share of the terms, and
+// a SIMD runs its waves one round after another: with one lane per line every
+// wave sums all p terms, with four lanes a quarter (11.2 against 28.6 ms per
+// round for 1,024 charges and 300 rows, ref rule).  Four lanes per line take
+// at most two rounds (two workgroups per CU) up to half a lane per SIMD slot
+// (n <= 32,768 on 256 CUs); beyond, one lane per line (one round up to 65,536
+// lines) is faster (at 40,000 / 60,000 lines: 28.7 / 28.6 ms, against 29.7 /
+// 38.2 ms with four lanes and 39.4 / 38.9 ms with two).
 uint32_t lanes64(const DeviceInfo& dev, uint32_t n)
 {
+    // EMTRACE_FP64_LANES = 1, 2 or 4 overrides the choice (the rows do not
+    // depend on it)
+    if (const char* env = std::getenv("EMTRACE_FP64_LANES")) {
+        const int v = std::atoi(env);
+        if (v == 1 || v == 2 || v == 4) return (uint32_t)v;
+    }
     const uint64_t simd_lanes = (uint64_t)dev.cus * 4 * 64;
-    if ((uint64_t)n * 2 <= simd_lanes) return 4;
-    if ((uint64_t)n < simd_lanes) return 2;
-    return 1;
+    return (uint64_t)n * 2 <= simd_lanes ? 4 : 1;
 }
 
 template <int RULE, int S>

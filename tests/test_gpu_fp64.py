@@ -50,16 +50,22 @@ CASES = [
     ("ragged_block", 7, 130, 3.0, 50, rules.angle()),
     ("multi_tile_plain", 5000, 128, 20.0, 30, rules.plain(10.0)),
     ("multi_tile_angle", 4100, 192, 20.0, 40, rules.angle()),
-    # 32,768 < n < 65,536 lines: two lanes per line (the cases above take four,
-    # the 65,536-line tests one); 67 charges leave a last pass of one term
+    # 32,768 < n lines: one lane per line (the cases above take four); two lanes
+    # per line, only by EMTRACE_FP64_LANES; 67 charges leave a last pass of one term
+    ("one_lane_plain", 67, 40000, 10.0, 30, rules.plain(10.0)),
     ("two_lanes_plain", 67, 40000, 10.0, 30, rules.plain(10.0)),
     ("two_lanes_angle", 67, 40000, 5.0, 40, rules.angle()),
+    ("four_lanes_angle", 67, 40000, 5.0, 40, rules.angle()),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_fp64_bitwise_equals_restatement(gpu, orc, case):
+def test_fp64_bitwise_equals_restatement(gpu, orc, case, monkeypatch):
     name, p, n, half, length, rule = case
+    if name.startswith("two_lanes"):
+        monkeypatch.setenv("EMTRACE_FP64_LANES", "2")
+    elif name.startswith("four_lanes"):
+        monkeypatch.setenv("EMTRACE_FP64_LANES", "4")
     rule = _as_f32_params(rule)
     w = gen.plain_config(p, n, half, length, seed=11)
     ch = w.charges.astype(np.float64)
