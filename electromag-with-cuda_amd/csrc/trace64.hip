@@ -280,80 +280,95 @@ __device__ __forceinline__ void sum_range(const double4* __restrict__ tile, uint
 template <int S, int J>
 __device__ __forceinline__ double from_lane(double v)
 {
-    static_assert(S == 2 || S == 4, "groups of 2 or 4 lanes");
-    // quad_perm selects: lane J of {0,1} and of {2,3}, or lane J of {0..3}
-    constexpr int ctrl = S == 4 ? J * 0x55 : (J | (J << 2) | ((2 + J) << 4) | ((2 + J) << 6));
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    static_assert(S == 2 || S == 4 || S == 8, "groups of 2, 4 or 8 lanes");
+    if constexpr (S == 8) {
+        return __shfl(v, J, 8);  // eight-lane groups span two quads: ds_bpermute
+    } else {
+        // quad_perm selects: lane J of {0,1} and of {2,3}, or lane J of {0..3}
+        constexpr int ctrl = S == 4 ? J * 0x55 : (J | (J << 2) | ((2 + J) << 4) | ((2 + J) << 6));
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)b, ctrl, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
+        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    }
+}
+
+template <int S, int J = 0>
+__device__ __forceinline__ void add_lanes_from(double& s, double v, uint32_t m)
+{
+    if constexpr (J < S) {
+        if (m > (uint32_t)J) {
+            s += from_lane<S, J>(v);
+            add_lanes_from<S, J + 1>(s, v, m);
+        }
+    }
 }
 
 // s += the terms of lanes 0 .. m-1 of the group, in lane order (m <= S)
 template <int S>
 __device__ __forceinline__ void add_in_order(double& s, double v, uint32_t m)
 {
-    if constexpr (S == 1) {
+    if constexpr (S == 1)
         s += v;
-    } else {
-        s += from_lane<S, 0>(v);
-        if (m > 1) s += from_lane<S, 1>(v);
-        if constexpr (S == 4) {
-            if (m > 2) s += from_lane<S, 2>(v);
-            if (m > 3) s += from_lane<S, 3>(v);
-        }
-    }
+    else
+        add_lanes_from<S>(s, v, m);
 }
 
-// Four-lane groups in the fast form: lane k of a group adds only component k
-// (lane 3 repeats component 2), so each lane adds 16 terms per pass instead of
-// reading all 48 term components through DPP moves.  The group's terms pass
-// through a per-wave LDS buffer: written at the end of a pass, read at the
-// start of the next, before that pass overwrites them (LDS operations of one
-// wave complete in program order, so a lane reads what the other lanes of its
-// group wrote one pass earlier).  The additions run in charge order, as in the
-// one-lane loop, threaded through the next pass's term stages; the three
-// totals are broadcast back to the group before the last charges of the tile.
+// Four- and eight-lane groups in the fast form: lane k of a group adds only
+// component k (lanes 3 .. S-1 repeat component 2), so each lane adds S U terms
+// per pass instead of reading all 3 S U term components from the other lanes.
+// The group's terms pass through a per-wave LDS buffer: written at the end of
+// a pass, read at the start of the next, before that pass overwrites them (LDS
+// operations of one wave complete in program order, so a lane reads what the
+// other lanes of its group wrote one pass earlier).  The additions run in
+// charge order, as in the one-lane loop, threaded through the next pass's term
+// stages; the three totals are broadcast back to the group before the last
+// charges of the tile.
 //
 // Buffer layout, in 16-byte chunks of two doubles: chunk (q, g, c) at
-// q * kSplitQ + 3 g + c holds elements 2q, 2q+1 of component c of group g,
+// q * split_q(S) + 3 g + c holds elements 2q, 2q+1 of component c of group g,
 // where element j U + u is lane j's term u.  A reader (g, c) takes chunks
-// q = 0 .. 2U-1 in turn; a quarter-wave's 12 distinct chunks are consecutive,
-// so at most two share a bank.  kSplitQ = 49 (not 48) staggers the writers'
+// q = 0 .. S U / 2 - 1 in turn; a quarter-wave's distinct chunks (12 for S = 4,
+// 6 for S = 8) are consecutive, so at most two share a bank.  The extra chunk
+// in split_q (49, not 48, for S = 4; 25 for S = 8) staggers the writers'
 // chunks (lane k writes q = 2k, 2k+1 for U = 4) across banks as well.
-constexpr int kSplitU = 4;                                      // terms per lane per pass
-constexpr uint32_t kSplitQ = 16 * 3 + 1;                         // chunks per q
-constexpr uint32_t kSplitBuf = 2 * kSplitU * kSplitQ * 2;        // doubles per wave
-constexpr uint32_t kSplitLdsBytes = (kBlock64 / 64) * kSplitBuf * sizeof(double);  // 24.5 KiB
-constexpr uint32_t kTileSplit = 1024;  // records staged at once: 32 KiB + 24.5 KiB, two per CU
+constexpr int kSplitU = 4;  // terms per lane per pass
+__host__ __device__ constexpr uint32_t split_q(int S) { return 64 / S * 3 + 1; }
+__host__ __device__ constexpr uint32_t split_buf(int S) { return S * kSplitU / 2 * split_q(S) * 2; }  // doubles per wave
+__host__ __device__ constexpr uint32_t split_lds_bytes(int S)
+{
+    return (kBlock64 / 64) * split_buf(S) * sizeof(double);  // 24.5 KiB (S = 4), 25 KiB (S = 8)
+}
+constexpr uint32_t kTileSplit = 1024;  // records staged at once: 32 KiB + ~25 KiB, two per CU
 
 // The previous pass's additions, threaded through this pass's term stages
-// (two per stage from stage 5 on, by when the LDS reads have landed): element e
-// of the pass is lane e % 4's term u = e / 4, added in e order.
+// (from stage 5 on, by when the LDS reads have landed): element e of the pass
+// is lane e % S's term u = e / S, added in e order.
+template <int S>
 struct SplitAdds {
     double& acc;
-    const double (&v)[4 * kSplitU];
+    const double (&v)[S * kSplitU];
     template <int I>
     __device__ __forceinline__ void at()
     {
-        constexpr int kFirst = 5, kPer = 2;
-        static_assert(kFirst + 4 * kSplitU / kPer <= 20, "the additions fit the plain rule's stages");
+        constexpr int kFirst = 5, kN = S * kSplitU, kPer = (kN + 14) / 15;  // within stages 5 .. 19
         if constexpr (I >= kFirst) {
 #pragma unroll
-            for (int e = (I - kFirst) * kPer; e < (I - kFirst + 1) * kPer && e < 4 * kSplitU; ++e)
-                acc += v[(e % 4) * kSplitU + e / 4];
+            for (int e = (I - kFirst) * kPer; e < (I - kFirst + 1) * kPer && e < kN; ++e)
+                acc += v[(e % S) * kSplitU + e / S];
         }
     }
 };
 
-template <int RULE>
+template <int RULE, int S>
 __device__ __forceinline__ uint32_t sum_passes_split(const double4* __restrict__ tile,
                                                      uint32_t count, uint32_t k, double px,
                                                      double py, double pz, double& sx, double& sy,
                                                      double& sz, double* __restrict__ xw)
 {
-    constexpr int S = 4, U = kSplitU;
-    constexpr uint32_t G = S * U;
+    static_assert(S == 4 || S == 8, "four- or eight-lane groups");
+    constexpr int U = kSplitU;
+    constexpr uint32_t G = S * U, Q = split_q(S);
     const uint32_t g = (threadIdx.x % 64) / S;
     const uint32_t comp = k < 3 ? k : 2;
     double acc = k == 0 ? sx : (k == 1 ? sy : sz);
@@ -365,13 +380,13 @@ __device__ __forceinline__ uint32_t sum_passes_split(const double4* __restrict__
         for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
             for (int u = 0; u < U; u += 2)
-                xc[((k * U + u) / 2) * kSplitQ + 3 * g + cc] = make_double2(e[u][cc], e[u + 1][cc]);
+                xc[((k * U + u) / 2) * Q + 3 * g + cc] = make_double2(e[u][cc], e[u + 1][cc]);
         __builtin_amdgcn_wave_barrier();
     };
     auto read = [&](double (&v)[S * U]) {
 #pragma unroll
         for (int q = 0; q < S * U / 2; ++q) {
-            const double2 t = xc[q * kSplitQ + 3 * g + comp];
+            const double2 t = xc[q * Q + 3 * g + comp];
             v[2 * q] = t.x;
             v[2 * q + 1] = t.y;
         }
@@ -401,7 +416,7 @@ __device__ __forceinline__ uint32_t sum_passes_split(const double4* __restrict__
         for (int u = 0; u < U; ++u) cn[u] = tile[nx + u * S + k];
         __builtin_amdgcn_sched_barrier(0);
         double e[U][3];
-        terms64<RULE, true, U>(c, px, py, pz, e, SplitAdds{acc, v});
+        terms64<RULE, true, U>(c, px, py, pz, e, SplitAdds<S>{acc, v});
         write(e);
 #pragma unroll
         for (int u = 0; u < U; ++u) c[u] = cn[u];
@@ -430,7 +445,8 @@ __device__ __forceinline__ void sum_range_group(const double4* __restrict__ tile
         constexpr int U = FAST ? kUnroll64Fast : kUnroll64;
         constexpr uint32_t G = S * U;  // charges per pass
         uint32_t i = 0;
-        if constexpr (S == 4 && FAST) i = sum_passes_split<RULE>(tile, count, k, px, py, pz, sx, sy, sz, xw);
+        if constexpr ((S == 4 || S == 8) && FAST)
+            i = sum_passes_split<RULE, S>(tile, count, k, px, py, pz, sx, sy, sz, xw);
         for (; i + G <= count; i += G) {
             double4 c[U];
 #pragma unroll
@@ -544,8 +560,8 @@ __global__ __launch_bounds__(kBlock64) void k_trace64(const double4* __restrict_
     st.steps = row0 - 1;
     st.alive = valid;
     const bool single = p <= tile;
-    // this wave's exchange buffers (four-lane groups), after the tile
-    double* const xw = S == 4 ? reinterpret_cast<double*>(lds64 + tile) + (threadIdx.x / 64) * kSplitBuf
+    // this wave's exchange buffer (four- and eight-lane groups), after the tile
+    double* const xw = S >= 4 ? reinterpret_cast<double*>(lds64 + tile) + (threadIdx.x / 64) * split_buf(S)
                               : nullptr;
     // the tile holds a charge outside the fast domain (staged tiles: every step)
     bool tile_out = sp.exact != 0;
@@ -644,20 +660,24 @@ StepP64 from_step(const em_step& st)
 // Lanes per line.  A wave's time is set by its lanes' share of the terms, and
 // a SIMD runs its waves one round after another: with one lane per line every
 // wave sums all p terms, with four lanes a quarter (11.2 against 28.6 ms per
-// round for 1,024 charges and 300 rows, ref rule).  Four lanes per line take
-// at most two rounds (two workgroups per CU) up to half a lane per SIMD slot
-// (n <= 32,768 on 256 CUs); beyond, one lane per line (one round up to 65,536
-// lines) is faster (at 40,000 / 60,000 lines: 28.7 / 28.6 ms, against 29.7 /
-// 38.2 ms with four lanes and 39.4 / 38.9 ms with two).
+// round for 1,024 charges and 300 rows, ref rule).  Eight lanes per line while
+// they fill at most one wave per SIMD (n <= 8,192 on 256 CUs: 6.9 against
+// 10.7 ms with four lanes at 8,192 lines; at 16,384, two waves per SIMD, 11.8
+// against 10.6 ms).  Four lanes per line up to half a lane per SIMD slot
+// (n <= 32,768: at most two rounds, two workgroups per CU); beyond, one lane per
+// line (one round up to 65,536 lines) is faster (at 40,000 / 60,000 lines:
+// 28.7 / 28.6 ms, against 29.7 / 38.2 ms with four lanes and 39.4 / 38.9 ms
+// with two).
 uint32_t lanes64(const DeviceInfo& dev, uint32_t n)
 {
     // EMTRACE_FP64_LANES = 1, 2 or 4 overrides the choice (the rows do not
     // depend on it)
     if (const char* env = std::getenv("EMTRACE_FP64_LANES")) {
         const int v = std::atoi(env);
-        if (v == 1 || v == 2 || v == 4) return (uint32_t)v;
+        if (v == 1 || v == 2 || v == 4 || v == 8) return (uint32_t)v;
     }
     const uint64_t simd_lanes = (uint64_t)dev.cus * 4 * 64;
+    if ((uint64_t)n * 8 <= simd_lanes) return 8;
     return (uint64_t)n * 2 <= simd_lanes ? 4 : 1;
 }
 
@@ -677,7 +697,9 @@ void launch64_rule(uint32_t S, size_t lds, hipStream_t stream, const double4* q,
 {
     const uint32_t per = kBlock64 / S;
     const dim3 grid((n + per - 1) / per);
-    if (S == 4)
+    if (S == 8)
+        launch64_s<RULE, 8>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
+    else if (S == 4)
         launch64_s<RULE, 4>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
     else if (S == 2)
         launch64_s<RULE, 2>(grid, lds, stream, q, p, x, y, z, n, len, pitch, sp, tile, steps, row0);
@@ -686,11 +708,11 @@ void launch64_rule(uint32_t S, size_t lds, hipStream_t stream, const double4* q,
 }
 
 // Charge records staged in LDS at once: all of them up to kTile64 (64 KiB, two
-// workgroups per CU), else tiles of half that; four-lane groups also hold their
-// exchange buffers (24.5 KiB), so their tiles stop at kTileSplit.
+// workgroups per CU), else tiles of half that; four- and eight-lane groups also
+// hold their exchange buffers (~25 KiB), so their tiles stop at kTileSplit.
 uint32_t tile64(uint32_t p, uint32_t S)
 {
-    const uint32_t cap = S == 4 ? kTileSplit : kTile64;
+    const uint32_t cap = S >= 4 ? kTileSplit : kTile64;
     return p <= cap ? std::max<uint32_t>(p, 1) : kTile64 / 2;
 }
 
@@ -703,7 +725,8 @@ int launch64(const DeviceInfo& dev, const double* charges, uint32_t p, double* x
         EM_FAIL(EM_ERR_INVALID, "row0 %u: the angle rule traces from row 1", row0);
     const uint32_t S = lanes64(dev, n);
     const uint32_t tile = tile64(p, S);
-    const size_t lds = (size_t)tile * sizeof(double4) + (S == 4 ? kSplitLdsBytes : 0);
+    const size_t lds = (size_t)tile * sizeof(double4) +
+                       (S == 8 ? split_lds_bytes(8) : S == 4 ? split_lds_bytes(4) : 0);
     const double4* q = reinterpret_cast<const double4*>(charges);
     // EMTRACE_FP64_EXACT=1: every term through the compiler's sqrt and division
     // expansions (the same bits; for tests and timing of the fast domain)

@@ -56,6 +56,10 @@ CASES = [
     ("two_lanes_plain", 67, 40000, 10.0, 30, rules.plain(10.0)),
     ("two_lanes_angle", 67, 40000, 5.0, 40, rules.angle()),
     ("four_lanes_angle", 67, 40000, 5.0, 40, rules.angle()),
+    # eight lanes per line (groups across two quads of lanes), single and staged tiles
+    ("eight_lanes_ref", 300, 512, 10.0, 200, rules.ref(10.0)),
+    ("eight_lanes_angle", 300, 448, 5.0, 300, rules.angle()),
+    ("eight_lanes_multi_tile_plain", 5000, 128, 20.0, 30, rules.plain(10.0)),
 ]
 
 
@@ -66,6 +70,8 @@ def test_fp64_bitwise_equals_restatement(gpu, orc, case, monkeypatch):
         monkeypatch.setenv("EMTRACE_FP64_LANES", "2")
     elif name.startswith("four_lanes"):
         monkeypatch.setenv("EMTRACE_FP64_LANES", "4")
+    elif name.startswith("eight_lanes"):
+        monkeypatch.setenv("EMTRACE_FP64_LANES", "8")
     rule = _as_f32_params(rule)
     w = gen.plain_config(p, n, half, length, seed=11)
     ch = w.charges.astype(np.float64)
@@ -159,16 +165,19 @@ def _fast_vs_exact(gpu, monkeypatch, q, s, length, rule, label):
     assert same.all() and torch.equal(fs, es)
 
 
-def test_fp64_four_lane_groups_equal_compiler_expansions_default_problem(gpu, monkeypatch):
+@pytest.mark.parametrize("lanes", ["4", "8"])
+def test_fp64_lane_groups_equal_compiler_expansions_default_problem(gpu, monkeypatch, lanes):
     """The reference application's default problem (1,024 charges, 16,384
-    lines, 2,500 rows, curvature step) on doubles: four lanes per line, one
-    staged tile, each lane adding one component of its group's terms; bit for
-    bit equal to the compiler's square roots and divisions with every lane
-    adding all terms (4.2e10 terms)."""
+    lines, 2,500 rows, curvature step) on doubles: four or eight lanes per
+    line, one staged tile, each lane adding one component of its group's terms;
+    bit for bit equal to the compiler's square roots and divisions with every
+    lane adding all terms (4.2e10 terms)."""
+    monkeypatch.setenv("EMTRACE_FP64_LANES", lanes)
     w = legacy.preset("default")
     q = torch.from_numpy(w.charges.astype(np.float64)).cuda()
     s = torch.from_numpy(w.seeds.astype(np.float64)).cuda()
-    _fast_vs_exact(gpu, monkeypatch, q, s, w.len, _as_f32_params(rules.ref(1.0)), "default problem")
+    _fast_vs_exact(gpu, monkeypatch, q, s, w.len, _as_f32_params(rules.ref(1.0)),
+                   f"default problem, {lanes} lanes")
 
 
 @pytest.mark.parametrize("rule", [rules.plain(10.0), rules.angle()], ids=["plain", "angle"])
