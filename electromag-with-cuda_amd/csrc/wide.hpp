@@ -9,7 +9,10 @@ namespace em {
 constexpr uint32_t kWideSlots = 16;     // lines a workgroup traces at once (default)
 constexpr int kWideMaxSlots = 32;       // ... at most (EMTRACE_WIDE_SLOTS)
 constexpr uint32_t kWideMaxCharges = 16384;  // charges a workgroup holds in registers
-constexpr uint32_t kWideSettle = 128;   // steps of the settling pass
+#ifndef EMTRACE_WIDE_SETTLE
+#define EMTRACE_WIDE_SETTLE 128
+#endif
+constexpr uint32_t kWideSettle = EMTRACE_WIDE_SETTLE;  // steps of the settling pass
 
 struct WideCtrl {
     uint32_t next_line;  // FRESH: next line to hand out
